@@ -1,0 +1,131 @@
+/*
+ * tpd_hip.h -- C ABI of the MI355X Gaussian rasterizer (libtpd_hip.so).
+ *
+ * This is the drop-in boundary for torpedo's Gaussian engine (torpedo/volumetric).  The
+ * reference exposes a C++ class, GaussianEngine (GaussianEngine.h:15-27), whose methods record
+ * Vulkan dispatches of Slang kernels; each entry point below replaces one of its operations and
+ * is cited next to its declaration.  The C++ API in include/torpedo/ (Scene, Camera,
+ * PerspectiveCamera, GaussianEngine) is built on these functions; Python binds them with ctypes.
+ *
+ * Conventions
+ *   - Every function returns TPD_OK (0) or a TPD_ERR_* code and never throws; the message of the
+ *     last failure on the calling thread is available from tpd_last_error().
+ *   - Pointers named d_* are device pointers (hipMalloc'd, or torch CUDA tensors); all other
+ *     array pointers are host memory, borrowed for the duration of the call.
+ *   - Device memory of a context is owned by the context.  One context per host thread.
+ *   - stream arguments are hipStream_t values; NULL selects the context's own stream.
+ *   - Matrices are 4x4 row-major float[16], as torpedo's mat4 (math/mat4.h:8-13).
+ */
+#ifndef TPD_HIP_H
+#define TPD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tpd_status {
+    TPD_OK = 0,
+    TPD_ERR_INVALID = 1,   /* bad argument */
+    TPD_ERR_HIP = 2,       /* HIP runtime error */
+    TPD_ERR_NO_SCENE = 3,  /* no point cloud uploaded */
+    TPD_ERR_NO_CAMERA = 4, /* no camera set */
+    TPD_ERR_CAPACITY = 5,  /* more than 2^30 - 1 tile/Gaussian pairs */
+    TPD_ERR_STATE = 6,     /* operation needs a previous step (e.g. read before render) */
+    TPD_ERR_NO_DEVICE = 7  /* no usable GPU */
+};
+
+typedef struct tpd_ctx tpd_ctx;
+
+/* GaussianPoint record (GaussianGeometry.h:10-30): 60 floats / 240 bytes. */
+#define TPD_GAUSSIAN_FLOATS 60
+/* Splat record as read back (splat.slang:33-39): color[3], tiles (uint32), texel[4], copac[4]. */
+#define TPD_SPLAT_FLOATS 12
+
+typedef struct tpd_config {
+    int32_t device;           /* HIP device ordinal */
+    uint32_t keep_unsorted;   /* non-zero: keep a copy of the pre-sort pairs for tpd_read_pairs */
+    uint32_t reserved[6];
+} tpd_config;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* tpd_last_error(void);
+
+/* Context<R>::create + bindEngine<GaussianEngine> (rendering/include/torpedo/rendering/Context.h:18-36,
+ * GaussianEngine.cpp:75-170): selects the device, creates a stream and the frame resources. */
+int tpd_create(tpd_ctx** out, const tpd_config* cfg);
+/* GaussianEngine::destroy (GaussianEngine.cpp:877-...) */
+int tpd_destroy(tpd_ctx* ctx);
+
+/* GaussianEngine::compile (GaussianEngine.cpp:359-399): uploads n GaussianPoint records.
+ * entity_idx (n entries, may be NULL = all entity 0) maps each Gaussian to one of n_entities
+ * model matrices (models, 16*n_entities floats, may be NULL = identity); sh_degree is clamped to
+ * 3 as in GaussianEngine.cpp:366-370.  n == 0 is accepted (renders black, as the reference
+ * warns and keeps going at GaussianEngine.cpp:362-365). */
+int tpd_upload(tpd_ctx* ctx, const float* gaussians, uint32_t n, const uint32_t* entity_idx, const float* models,
+               uint32_t n_entities, uint32_t sh_degree);
+
+/* Same, but the n x 60 float records already live in device memory (borrowed, not copied; they
+ * must stay valid while the context renders from them).  Used by the autograd wrapper. */
+int tpd_bind_device_scene(tpd_ctx* ctx, const float* d_gaussians, uint32_t n, uint32_t sh_degree);
+
+/* TransformHost::transform (rendering/src/TransformHost.cpp:3-12): sets one entity's model matrix. */
+int tpd_set_transform(tpd_ctx* ctx, uint32_t entity, const float model[16]);
+
+/* GaussianEngine::updateCameraBuffer (GaussianEngine.cpp:764-775) + the render target size
+ * (initRenderer / onFramebufferResize, GaussianEngine.cpp:206-220). */
+int tpd_set_camera(tpd_ctx* ctx, const float view[16], const float proj_view[16], const float focal_ndc[2],
+                   uint32_t width, uint32_t height);
+
+/* GaussianEngine::rasterFrame (GaussianEngine.cpp:621-712): preprocess, scan + duplicate, radix
+ * sort, ranges, blend.  Blocks once on the pair count, as the reference does at :665-670.
+ * Produces the FP32 image (H x W x 3) and the RGBA8 image on the device. */
+int tpd_render(tpd_ctx* ctx, void* stream);
+/* Waits for all work of the context's stream (Engine::waitIdle, rendering/include/torpedo/rendering/Engine.h:19). */
+int tpd_synchronize(tpd_ctx* ctx);
+
+/* ----- readbacks (the reference has none: its HeadlessRenderer cannot read pixels back) ----- */
+int tpd_get_pair_count(tpd_ctx* ctx, uint32_t* pairs);
+int tpd_read_image_f32(tpd_ctx* ctx, float* rgb);          /* H*W*3 floats */
+int tpd_read_image_rgba8(tpd_ctx* ctx, uint8_t* rgba);     /* H*W*4 bytes  */
+int tpd_read_splats(tpd_ctx* ctx, float* splats);          /* N*12 floats, tiles = per-Gaussian count */
+int tpd_read_tiles(tpd_ctx* ctx, uint32_t* counts, uint32_t* offsets); /* N each (offsets may be NULL) */
+/* Sorted (tile|depth) keys and Gaussian indices; with unsorted != 0 the keygen order instead
+ * (requires tpd_config.keep_unsorted).  capacity is the length of keys/vals. */
+int tpd_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capacity, int unsorted);
+int tpd_read_ranges(tpd_ctx* ctx, uint32_t* ranges);       /* 2 * tiles uint32: [start, end) per tile */
+/* Device pointer of the FP32 image (valid until the next render or resize). */
+int tpd_device_image(tpd_ctx* ctx, float** d_rgb);
+
+/* ----- stage timing (the reference has none; SurfaceRenderer only measures whole frames) -----
+ * With profiling on, tpd_render records HIP events on its stream between stages; ms[0..5]
+ * receive the last frame's preprocess+scan+duplicate, sort histograms, all radix passes, ranges,
+ * blend and whole-frame times in milliseconds; *passes the number of radix passes. */
+int tpd_set_profiling(tpd_ctx* ctx, int enable);
+int tpd_get_stage_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
+
+/* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----
+ * Sorts n (key, value) pairs by key (stable); key bits at and above key_bits must be zero.  d_keys_alt /
+ * d_vals_alt are scratch of the same size; *in_alt is set to 1 when the result is there. */
+int tpd_sort_pairs(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_keys_alt, uint32_t* d_vals_alt,
+                   uint32_t n, uint32_t key_bits, void* stream, int* in_alt);
+
+/* ----- host camera math (torpedo/math, rendering/src/Camera.cpp, extension/src/PerspectiveCamera.cpp) ----- */
+/* Camera::lookAt (Camera.cpp:3-16) */
+void tpd_look_at(const float eye[3], const float center[3], const float up[3], float view[16]);
+/* PerspectiveCamera::updateProjectionMatrix (PerspectiveCamera.cpp:9-23); fy = 1/tan(fov/2) */
+void tpd_perspective(uint32_t width, uint32_t height, float fy, float znear, float zfar, float proj[16]);
+/* PerspectiveCamera::setVerticalFov (PerspectiveCamera.cpp:3-7) */
+float tpd_fy_from_fov(float degrees);
+/* math::mul (mat4.h:256-264), compensated */
+void tpd_mat4_mul(const float a[16], const float b[16], float out[16]);
+/* updateCameraBuffer's host part (GaussianEngine.cpp:764-768): projView = proj * view, focal = (P00, P11) */
+void tpd_camera_upload(const float view[16], const float proj[16], float proj_view[16], float focal_ndc[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPD_HIP_H */

@@ -1,0 +1,157 @@
+"""Forward path on the GPU (through the C ABI) against the CPU oracle.
+
+Bar: splats, tile counts, offsets, keys, values and ranges bit-exact; the FP32 image bit-exact
+with the oracle (the test asserts max-abs <= 1e-5 and reports the number of differing pixels);
+RGBA8 exact.  The image's error against the FP64 twin is printed per case.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import c1_camera, small_scene
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from torpedo_amd.engine import GaussianEngine
+    e = GaussianEngine(keep_unsorted=True)
+    yield e
+    e.close()
+
+
+def gpu_render(eng, g, cam, deg):
+    eng.uploadRecords(g, deg)
+    eng.setCamera(cam.view, cam.proj_view, cam.focal_ndc, cam.width, cam.height)
+    eng.render()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def compare_frame(eng, orc, g, cam, deg, name, check_f64=False):
+    say(f"[forward] {name}: N={g.shape[0]} {cam.width}x{cam.height} deg={deg}")
+    gpu_render(eng, g, cam, deg)
+    ref = orc.render(g, cam, deg)
+    sp = eng.readSplats()
+    for f in ("color", "texel", "copac"):
+        diff = np.nonzero(np.any(bits(sp[f]) != bits(ref["splats"][f]), axis=1))[0]
+        assert diff.size == 0, (name, f, diff[:5], sp[f][diff[:3]], ref["splats"][f][diff[:3]])
+    assert np.array_equal(sp["tiles"], ref["splats"]["tiles"]), name
+    counts, offs = eng.readTiles()
+    assert np.array_equal(offs, ref["offsets"]), name
+    assert eng.pairCount == ref["total"], name
+    uk, uv = eng.readPairs(unsorted=True)
+    assert np.array_equal(uk, ref["keys_unsorted"]) and np.array_equal(uv, ref["vals_unsorted"]), name
+    k, v = eng.readPairs()
+    assert np.array_equal(k, ref["keys"]) and np.array_equal(v, ref["vals"]), name
+    assert np.array_equal(eng.readRanges(), ref["ranges"]), name
+    img = eng.readImage()
+    err = np.abs(img - ref["image"])
+    ndiff = int(np.count_nonzero(err))
+    say(f"[forward] {name}: P={ref['total']} image max-abs vs oracle {err.max():.3g} ({ndiff} values differ)")
+    assert err.max() <= IMG_TOL, name
+    assert np.array_equal(eng.readRGBA8(), ref["rgba8"]), name
+    if check_f64:
+        _, img64 = orc.render_f64(g, cam, deg)
+        e64 = np.abs(img.astype(np.float64) - img64)
+        say(f"[forward] {name}: image vs FP64 twin max-abs {e64.max():.3g}, mean {e64.mean():.3g}")
+    return ref
+
+
+def test_plain_forward_100k_720p(eng, orc):
+    from torpedo_amd import scenes
+    g = scenes.uniform_scene(100_000, seed=1)
+    compare_frame(eng, orc, g, c1_camera(orc), 3, "uniform-100k-1280x720", check_f64=True)
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_sh_degrees(eng, orc, deg):
+    g = small_scene(4000, seed=20 + deg)
+    compare_frame(eng, orc, g, c1_camera(orc, 320, 180), deg, f"sh-degree-{deg}")
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (17, 33), (64, 64), (250, 130)])
+def test_ragged_image_sizes(eng, orc, w, h):
+    g = small_scene(3000, seed=w * 7 + h, log_scale_mean=-3.5)
+    compare_frame(eng, orc, g, c1_camera(orc, w, h), 3, f"size-{w}x{h}")
+
+
+def test_edge_scenes(eng, orc):
+    from torpedo_amd.engine import gaussian_point
+    cam = c1_camera(orc, 160, 96)
+    # a single big Gaussian covering every tile
+    big = gaussian_point([0, 0, 0], 0.9, (0, 0, 0, 1), (2.0, 2.0, 2.0), np.full(48, 0.1, np.float32))[None]
+    compare_frame(eng, orc, big, cam, 3, "one-giant-gaussian")
+    # everything culled (behind the camera)
+    g = small_scene(1000, seed=3)
+    g[:, 2] -= 10.0
+    ref = compare_frame(eng, orc, g, cam, 3, "all-culled")
+    assert ref["total"] == 0 and np.all(eng.readImage() == 0)
+    # empty scene renders black (the reference warns and renders nothing)
+    say("[forward] empty-scene")
+    gpu_render(eng, np.zeros((0, 60), np.float32), cam, 3)
+    assert eng.pairCount == 0 and np.all(eng.readImage() == 0)
+    assert np.all(eng.readRGBA8()[..., 3] == 255) and np.all(eng.readRGBA8()[..., :3] == 0)
+    assert np.all(eng.readRanges() == 0)
+    # opacity 0 and 1, duplicates with identical depth (stable order by index)
+    g = small_scene(2000, seed=4)
+    g[:500, 3] = 0.0
+    g[500:1000, 3] = 1.0
+    g[1000:1500] = g[1500:2000]
+    compare_frame(eng, orc, g, cam, 3, "opacity-extremes-and-duplicates")
+
+
+def test_buffer_growth_replay(eng, orc):
+    cam = c1_camera(orc, 320, 240)
+    compare_frame(eng, orc, small_scene(500, seed=8), cam, 3, "growth-small")
+    compare_frame(eng, orc, small_scene(60000, seed=9, log_scale_mean=-2.5), cam, 3, "growth-large")
+
+
+def test_entities_and_transforms(eng, orc):
+    from torpedo_amd.engine import PerspectiveCamera, Scene, Settings, gaussian_point
+    a = small_scene(3000, seed=30)
+    b = small_scene(2000, seed=31)
+    single = gaussian_point([0.2, 0.1, 0.0], 0.8, (0, 0, 0, 1), (0.2, 0.1, 0.3), np.full(48, 0.05, np.float32))
+    scene = Scene()
+    ea = scene.add(a)
+    eb = scene.add(b)
+    es = scene.add(single)
+    say("[forward] entities-and-transforms")
+    eng.compile(scene, Settings(3))
+    T1 = np.array([[1, 0, 0, 0.3], [0, 1, 0, -0.2], [0, 0, 1, 0.1], [0, 0, 0, 1]], np.float32)
+    c, s = np.cos(0.3), np.sin(0.3)
+    T2 = np.array([[c, 0, s, 0], [0, 1, 0, 0.1], [-s, 0, c, 0], [0, 0, 0, 1]], np.float32)
+    eng.getTransformHost().transform(eb, T1)
+    eng.getTransformHost().transform(es, T2)
+    eng.getTransformHost().transform(12345, T2)  # unknown entity: ignored
+    cam = PerspectiveCamera(320, 180)
+    cam.lookAt([0.3, -0.5, -3.0], [0, 0, 0], [0, -1, 0])
+    eng.rasterFrame(cam)
+    view, pv, fo = cam.uniforms()
+    ocam = orc.Camera(view, pv, fo, 320, 180)
+    models = np.stack([np.eye(4, dtype=np.float32), T1, T2]).reshape(3, 16)
+    ref = orc.render(scene.dataAll(), ocam, 3, ent_idx=scene.entityIndices(), models=models)
+    k, v = eng.readPairs()
+    assert np.array_equal(k, ref["keys"]) and np.array_equal(v, ref["vals"])
+    assert np.abs(eng.readImage() - ref["image"]).max() <= IMG_TOL
+    assert ea == 0
+
+
+def test_cluster_scene_1m_1080p(eng, orc):
+    from torpedo_amd import scenes
+    g = scenes.cluster_scene(1_000_000, seed=2)
+    cam = orc.make_camera([0, 0, -8], [0, 0, 0], [0, -1, 0], 1920, 1080)
+    ref = compare_frame(eng, orc, g, cam, 3, "clusters-1M-1920x1080")
+    k = ref["keys"]
+    assert np.all(k[1:] >= k[:-1])

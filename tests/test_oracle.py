@@ -1,0 +1,296 @@
+"""The CPU oracle pinned against what fixes it independently of its own formulas.
+
+The reference ships no tests, fixtures or golden vectors for this path, and its Slang pipeline
+cannot run here, so each oracle stage is pinned by:
+  camera      the reference's own camera/math code compiled from /root/reference (oracle/_ref),
+              bit for bit, plus closed forms;
+  exp         float64 exp within 2 ulp over the blend's domain;
+  preprocess  closed forms (isotropic Gaussian on the optical axis, SH bands, the band-3 typo),
+              the FP64 twin, invariants (conic * cov = I, radius, tile-rectangle area);
+  scan        numpy cumsum;   duplicate: brute-force rectangle enumeration in Python;
+  sort        numpy's stable argsort;   ranges: brute force;
+  blend       single-Gaussian closed form, alpha clamp / 1/255 skip / T < 1e-4 termination cases,
+              the FP64 twin.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import c1_camera, small_scene
+
+SQRT3 = float(np.float32(math.sqrt(3.0)))
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+# ------------------------------------------------------------------ camera
+
+def test_look_at_closed_form(orc):
+    v = orc.look_at([0, 0, -3], [0, 0, 0], [0, -1, 0])
+    expect = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 3], [0, 0, 0, 1]], np.float32)
+    assert np.array_equal(np.abs(v), np.abs(expect))  # -0 entries allowed
+    p = orc.perspective(1280, 720)
+    assert p[1, 1] == np.float32(SQRT3)
+    assert p[0, 0] == np.float32(np.float32(SQRT3) / np.float32(np.float32(1280) / np.float32(720)))
+    assert p[3, 2] == 1.0 and p[3, 3] == 0.0
+    near, far = np.float32(0.01), np.float32(100.0)
+    assert p[2, 2] == near / (near - far)
+    assert p[2, 3] == (near * far) / (far - near)
+    assert abs(orc.fy_from_fov(60.0) - math.sqrt(3.0)) < 1e-6
+
+
+def test_camera_matches_reference_build(orc):
+    ref = orc.ref_camera()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference at build time)")
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        eye = rng.normal(size=3).astype(np.float32) * 5
+        cen = rng.normal(size=3).astype(np.float32)
+        up = rng.normal(size=3).astype(np.float32)
+        a = orc.look_at(eye, cen, up).ravel()
+        b = np.zeros(16, np.float32)
+        ref.tpr_look_at(eye, cen, up, b)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        A = rng.normal(size=16).astype(np.float32)
+        B = rng.normal(size=16).astype(np.float32)
+        m1 = orc.mat4_mul(A, B).ravel()
+        m2 = np.zeros(16, np.float32)
+        ref.tpr_mat4_mul(A, B, m2)
+        assert np.array_equal(m1.view(np.uint32), m2.view(np.uint32))
+    for w, h in [(1280, 720), (1920, 1080), (256, 256), (1600, 900), (333, 77)]:
+        for fov in [0.0, 45.0, 60.0, 90.0]:
+            r = np.zeros(16, np.float32)
+            ref.tpr_perspective(w, h, fov, r)
+            fy = SQRT3 if fov == 0.0 else orc.fy_from_fov(fov)
+            o = orc.perspective(w, h, fy).ravel()
+            assert np.array_equal(o.view(np.uint32), r.view(np.uint32)), (w, h, fov)
+
+
+def test_compensated_mul_is_accurate(orc):
+    # The compensated product is within a few ulp of the exact (float64) product.
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        A = rng.normal(size=(4, 4)).astype(np.float32)
+        B = rng.normal(size=(4, 4)).astype(np.float32)
+        exact = A.astype(np.float64) @ B.astype(np.float64)
+        got = orc.mat4_mul(A, B)
+        assert np.all(np.abs(got - exact) <= 4 * np.spacing(np.abs(exact).astype(np.float32)) + 1e-30)
+
+
+# ------------------------------------------------------------------ exp
+
+def test_soft_exp_accuracy(orc):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([-rng.uniform(0, 86, 200000), -rng.uniform(0, 6, 200000), np.linspace(-86, 0, 10001),
+                        [0.0, -0.0, -1e-30, -1e-7]]).astype(np.float32)
+    y = orc.expf(x)
+    ref = np.exp(x.astype(np.float64))
+    rel = np.abs(y.astype(np.float64) - ref) / ref
+    assert rel.max() < 2.5 * 2.0 ** -23, rel.max()
+    assert orc.expf([0.0])[0] == 1.0
+    assert orc.expf([-87.0])[0] == 0.0 and orc.expf([np.nan])[0] == 0.0
+    xs = np.sort(x)
+    assert np.all(np.diff(orc.expf(xs)) >= 0)  # monotone on the sample
+
+
+# ------------------------------------------------------------------ preprocess
+
+def _one(pos, scale, quat=(0, 0, 0, 1), opacity=0.5, sh=None):
+    from torpedo_amd.engine import gaussian_point
+    return gaussian_point(pos, opacity, quat, list(scale) + [1.0], sh)[None, :]
+
+
+def test_isotropic_on_axis_closed_form(orc):
+    cam = c1_camera(orc)
+    s = 0.05
+    sp = orc.preprocess(_one([0, 0, 0], [s, s, s]), cam, 0)
+    f = 0.5 * 720 * math.sqrt(3.0)
+    a = (f * s / 3.0) ** 2 + 0.3
+    ca, cb, cc = sp["copac"][0, :3]
+    assert abs(ca - 1 / a) < 1e-6 * (1 / a) and abs(cc - 1 / a) < 1e-6 * (1 / a) and cb == 0
+    r = math.ceil(3 * math.sqrt(a + math.sqrt(0.1)))
+    assert sp["texel"][0, 3] == r
+    assert sp["texel"][0, 0] == (1280 - 1) / 2 and sp["texel"][0, 1] == (720 - 1) / 2
+    assert sp["texel"][0, 2] == 3.0
+    x0, y0, x1, y1 = sp["rect"][0]
+    assert (x1 - x0) * (y1 - y0) == sp["tiles"][0]
+    px, py = (1280 - 1) / 2, (720 - 1) / 2
+    assert x0 == max(0, int((px - r) / 16)) and x1 == min(80, int((px + r + 15) / 16))
+    assert y0 == max(0, int((py - r) / 16)) and y1 == min(45, int((py + r + 15) / 16))
+
+
+def test_sh_bands_and_band3_typo(orc):
+    cam = c1_camera(orc)
+    c0 = 0.28209479177387814
+    c1 = 0.4886025119029199
+    s12 = 0.3731763325901154 * 1.0 * 2.0  # z = 1, x = y = 0: C3[3] z (2 zz - 3 xx - 3 yy)
+    sh = np.zeros(48, np.float32)
+    sh[0:3] = [1.0, -1.0, 0.25]
+    col = orc.preprocess(_one([0, 0, 0], [0.05] * 3, sh=sh), cam, 0)["color"][0]
+    np.testing.assert_allclose(col, np.maximum(c0 * sh[0:3] + 0.5, 0), rtol=1e-6)
+    # band 1: feature 2 multiplies z (= 1 on the axis)
+    sh = np.zeros(48, np.float32)
+    sh[3 + 0 * 15 + 1] = 0.4
+    col = orc.preprocess(_one([0, 0, 0], [0.05] * 3, sh=sh), cam, 1)["color"][0]
+    np.testing.assert_allclose(col, [0.5 + c1 * 0.4, 0.5, 0.5], rtol=1e-6)
+    # band 3 with every coefficient zero: the typo adds C3[3] z (...) itself
+    col = orc.preprocess(_one([0, 0, 0], [0.05] * 3), cam, 3)["color"][0]
+    np.testing.assert_allclose(col, [0.5 + s12] * 3, rtol=1e-6)
+    # ... and feature 12 is added, not multiplied
+    sh = np.zeros(48, np.float32)
+    sh[3 + 2 * 15 + 11] = 0.3
+    col = orc.preprocess(_one([0, 0, 0], [0.05] * 3, sh=sh), cam, 3)["color"][0]
+    np.testing.assert_allclose(col, [0.5 + s12, 0.5 + s12, 0.5 + s12 + 0.3], rtol=1e-6)
+    # degree 2 ignores band 3
+    col = orc.preprocess(_one([0, 0, 0], [0.05] * 3, sh=sh), cam, 2)["color"][0]
+    np.testing.assert_allclose(col, [0.5, 0.5, 0.5], rtol=1e-6)
+
+
+def test_culling_cases(orc):
+    cam = c1_camera(orc)
+    g = np.concatenate([
+        _one([0, 0, -3.5], [0.05] * 3),      # behind the camera
+        _one([10, 0, 0], [0.05] * 3),        # outside the 1.3 guard band in x
+        _one([0, 0, 200], [0.05] * 3),       # beyond far
+        _one([0, 0, -2.995], [0.05] * 3),    # in front of near plane (z < 0.01)
+        _one([1.2, 0, 0], [0.05] * 3),       # inside the guard band but off-screen (x = 1.3 w border)
+        _one([0, 0, 0], [0.05] * 3),         # visible
+    ])
+    sp = orc.preprocess(g, cam, 3)
+    assert list(sp["tiles"][:4]) == [0, 0, 0, 0]
+    assert np.all(sp["texel"][:4, 3] == 0)
+    assert sp["tiles"][5] > 0
+
+
+def test_preprocess_invariants_and_fp64_twin(orc):
+    cam = c1_camera(orc)
+    g = small_scene(20000, seed=11)
+    sp = orc.preprocess(g, cam, 3)
+    vis = sp["tiles"] > 0
+    assert vis.mean() > 0.5
+    x0, y0, x1, y1 = sp["rect"][vis].T
+    assert np.array_equal(((x1 - x0) * (y1 - y0)).astype(np.uint32), sp["tiles"][vis])
+    # conic is the inverse of the 2D covariance: recover cov from conic and check the radius rule
+    a, b, c = [sp["copac"][vis, k].astype(np.float64) for k in range(3)]
+    det_c = a * c - b * b
+    cxx, cxy, cyy = c / det_c, -b / det_c, a / det_c
+    mid = 0.5 * (cxx + cyy)
+    lam = mid + np.sqrt(np.maximum(0.1, mid * mid - (cxx * cyy - cxy * cxy)))
+    r = np.ceil(3 * np.sqrt(lam))
+    assert np.mean(r == sp["texel"][vis, 3]) > 0.999
+    assert np.all(cxx > 0.3 - 1e-3) and np.all(cyy > 0.3 - 1e-3)
+    # FP64 twin through the whole frame: the FP32 image stays close to the FP64 one
+    _, img64 = orc.render_f64(g, cam, 3)
+    img = orc.render(g, cam, 3)["image"]
+    err = np.abs(img.astype(np.float64) - img64)
+    assert np.quantile(err, 0.999) < 1e-4
+
+
+# ------------------------------------------------------------------ scan, duplicate, sort, ranges
+
+def test_scan_duplicate_sort_ranges_against_brute_force(orc):
+    cam = orc.make_camera([0, 0, -3], [0, 0, 0], [0, -1, 0], 160, 96)
+    g = small_scene(3000, seed=5, log_scale_mean=-3.5)
+    sp = orc.preprocess(g, cam, 3)
+    off, total = orc.scan(sp["tiles"])
+    assert total == int(sp["tiles"].astype(np.int64).sum())
+    assert np.array_equal(off, (np.cumsum(sp["tiles"].astype(np.int64)) - sp["tiles"]).astype(np.uint32))
+    keys, vals = orc.duplicate(sp["texel"], off, total, 160, 96)
+    gx = 10
+    bk, bv = [], []
+    for i in range(g.shape[0]):
+        if sp["texel"][i, 3] <= 0:
+            continue
+        x0, y0, x1, y1 = sp["rect"][i]
+        depth = int(sp["texel"][i, 2:3].view(np.uint32)[0])
+        for y in range(y0, y1):
+            for x in range(x0, x1):
+                bk.append(((y * gx + x) << 32) | depth)
+                bv.append(i)
+    assert np.array_equal(keys, np.array(bk, np.uint64)) and np.array_equal(vals, np.array(bv, np.uint32))
+    sk, sv = orc.sort_pairs(keys, vals)
+    o = np.argsort(keys, kind="stable")
+    assert np.array_equal(sk, keys[o]) and np.array_equal(sv, vals[o])
+    rng = orc.ranges(sk, cam.ntiles)
+    tiles_of = (sk >> np.uint64(32)).astype(np.int64)
+    for t in range(cam.ntiles):
+        idx = np.nonzero(tiles_of == t)[0]
+        if idx.size == 0:
+            assert tuple(rng[t]) == (0, 0)
+        else:
+            assert tuple(rng[t]) == (idx[0], idx[-1] + 1)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 255, 65535, 70001, 300000])
+def test_sort_matches_stable_argsort(orc, n):
+    rng = np.random.default_rng(n)
+    keys = (rng.integers(0, 50, n, dtype=np.uint64) << np.uint64(32)) | rng.integers(0, 8, n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv = orc.sort_pairs(keys, vals, nthreads=4)
+    o = np.argsort(keys, kind="stable")
+    assert np.array_equal(sk, keys[o]) and np.array_equal(sv, vals[o])
+
+
+# ------------------------------------------------------------------ blend
+
+def _splats(entries):
+    """entries: list of (x, y, depth, conic(a,b,c), opacity, rgb)."""
+    n = len(entries)
+    sp = dict(color=np.zeros((n, 3), np.float32), texel=np.zeros((n, 4), np.float32),
+              copac=np.zeros((n, 4), np.float32))
+    for i, (x, y, z, con, o, rgb) in enumerate(entries):
+        sp["texel"][i] = [x, y, z, 10.0]
+        sp["copac"][i] = list(con) + [o]
+        sp["color"][i] = rgb
+    return sp
+
+
+def test_blend_single_gaussian_closed_form(orc):
+    sp = _splats([(5.0, 7.0, 1.0, (0.5, 0.1, 0.25), 0.8, (0.2, 0.6, 1.0))])
+    rng = np.array([[0, 1]], np.uint32)
+    img, rgba = orc.blend(sp, np.zeros(1, np.uint32), rng, 16, 16)
+    for py in range(16):
+        for px in range(16):
+            dx, dy = 5.0 - px, 7.0 - py
+            power = -0.5 * (0.5 * dx * dx + 0.25 * dy * dy) - 0.1 * dx * dy
+            alpha = min(0.99, 0.8 * math.exp(power)) if power <= 0 else 0.0
+            if alpha < 1 / 255:
+                alpha = 0.0
+            expect = np.array([0.2, 0.6, 1.0]) * alpha
+            np.testing.assert_allclose(img[py, px], expect, rtol=2e-6, atol=1e-7)
+    assert np.all(rgba[..., 3] == 255)
+    assert rgba[7, 5, 2] == round(0.8 * 255)
+
+
+def test_blend_clamp_skip_and_termination(orc):
+    # three opaque Gaussians on one pixel: alpha clamps to 0.99, T = 0.01 after the first, and
+    # the second would drop T below 1e-4, so the pixel stops there (tested before blending).
+    sp = _splats([(3.0, 3.0, 1.0, (1, 0, 1), 1.0, (1, 0, 0)),
+                  (3.0, 3.0, 2.0, (1, 0, 1), 1.0, (0, 1, 0)),
+                  (3.0, 3.0, 3.0, (1, 0, 1), 1.0, (0, 0, 1))])
+    img, _ = orc.blend(sp, np.arange(3, dtype=np.uint32), np.array([[0, 3]], np.uint32), 16, 16)
+    assert img[3, 3, 0] == np.float32(0.99) and img[3, 3, 1] == 0 and img[3, 3, 2] == 0
+    # a faint Gaussian below 1/255 is skipped entirely
+    sp = _splats([(3.0, 3.0, 1.0, (1, 0, 1), 0.003, (1, 1, 1))])
+    img, _ = orc.blend(sp, np.zeros(1, np.uint32), np.array([[0, 1]], np.uint32), 16, 16)
+    assert np.all(img == 0)
+    # two half-opaque Gaussians: C = c1 a + c2 a (1 - a)
+    sp = _splats([(3.0, 3.0, 1.0, (1, 0, 1), 0.5, (1, 0, 0)),
+                  (3.0, 3.0, 2.0, (1, 0, 1), 0.5, (0, 1, 0))])
+    img, _ = orc.blend(sp, np.arange(2, dtype=np.uint32), np.array([[0, 2]], np.uint32), 16, 16)
+    np.testing.assert_allclose(img[3, 3], [0.5, 0.25, 0.0], rtol=1e-7)
+
+
+def test_full_frame_stage_by_stage_equals_one_call(orc):
+    cam = c1_camera(orc, 320, 180)
+    g = small_scene(5000, seed=2)
+    r = orc.render(g, cam, 3)
+    P, img, rgba, _ = orc.render_timed(g, cam, 3)
+    assert P == r["total"]
+    assert np.array_equal(img, r["image"]) and np.array_equal(rgba, r["rgba8"])

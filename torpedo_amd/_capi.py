@@ -1,0 +1,100 @@
+"""ctypes binding of libtpd_hip.so (declarations in include/tpd_hip.h).
+
+There is no fallback: if the library has not been built (``python -m torpedo_amd.build``) or
+cannot be loaded, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtpd_hip.so")
+
+TPD_OK = 0
+STATUS = {0: "TPD_OK", 1: "TPD_ERR_INVALID", 2: "TPD_ERR_HIP", 3: "TPD_ERR_NO_SCENE", 4: "TPD_ERR_NO_CAMERA",
+          5: "TPD_ERR_CAPACITY", 6: "TPD_ERR_STATE", 7: "TPD_ERR_NO_DEVICE"}
+
+# Every symbol include/tpd_hip.h declares, with (restype, argtypes).
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int
+_f32 = ctypes.c_float
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class TpdConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("keep_unsorted", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
+
+
+SIGNATURES = {
+    "tpd_last_error": (ctypes.c_char_p, []),
+    "tpd_create": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(TpdConfig)]),
+    "tpd_destroy": (_i32, [_vp]),
+    "tpd_upload": (_i32, [_vp, _vp, _u32, _vp, _vp, _u32, _u32]),
+    "tpd_bind_device_scene": (_i32, [_vp, _vp, _u32, _u32]),
+    "tpd_set_transform": (_i32, [_vp, _u32, _vp]),
+    "tpd_set_camera": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32]),
+    "tpd_render": (_i32, [_vp, _vp]),
+    "tpd_synchronize": (_i32, [_vp]),
+    "tpd_get_pair_count": (_i32, [_vp, ctypes.POINTER(_u32)]),
+    "tpd_read_image_f32": (_i32, [_vp, _vp]),
+    "tpd_read_image_rgba8": (_i32, [_vp, _vp]),
+    "tpd_read_splats": (_i32, [_vp, _vp]),
+    "tpd_read_tiles": (_i32, [_vp, _vp, _vp]),
+    "tpd_read_pairs": (_i32, [_vp, _vp, _vp, _u32, _i32]),
+    "tpd_read_ranges": (_i32, [_vp, _vp]),
+    "tpd_device_image": (_i32, [_vp, ctypes.POINTER(_vp)]),
+    "tpd_set_profiling": (_i32, [_vp, _i32]),
+    "tpd_get_stage_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
+    "tpd_sort_pairs": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(_i32)]),
+    "tpd_look_at": (None, [_vp, _vp, _vp, _vp]),
+    "tpd_perspective": (None, [_u32, _u32, _f32, _f32, _f32, _vp]),
+    "tpd_fy_from_fov": (_f32, [_f32]),
+    "tpd_mat4_mul": (None, [_vp, _vp, _vp]),
+    "tpd_camera_upload": (None, [_vp, _vp, _vp, _vp]),
+}
+
+_LIB = None
+
+
+class TpdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+LOADED_AFTER_TORCH = False
+
+
+def lib():
+    """The loaded library.  If torch is used in the same process it must be imported first: torch
+    carries its own HIP runtime, and the library then binds to it instead of loading a second one."""
+    global _LIB, LOADED_AFTER_TORCH
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m torpedo_amd.build`")
+        import sys
+        LOADED_AFTER_TORCH = "torch" in sys.modules
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code: int) -> None:
+    if code != TPD_OK:
+        raise TpdError(code, lib().tpd_last_error().decode())
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)

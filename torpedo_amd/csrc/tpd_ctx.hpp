@@ -1,0 +1,82 @@
+// The context behind the opaque tpd_ctx handle, and the host-side helpers shared by the C-ABI
+// translation units.
+#pragma once
+
+#include "tpd_kernels.hpp"
+#include "../../include/tpd_hip.h"
+
+#include <string>
+#include <vector>
+
+struct tpd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool keep_unsorted = false;
+
+    // scene (GaussianEngine::compile)
+    float* d_gauss = nullptr;  // N x 60
+    bool own_gauss = true;
+    uint32_t n = 0, sh_degree = 3;
+    uint32_t* d_ent_idx = nullptr;
+    float* d_models = nullptr;
+    uint32_t n_ent = 1;
+    std::vector<float> h_models;
+    bool have_scene = false;
+
+    // camera
+    tpd::CameraParams cam{};
+    bool have_camera = false;
+
+    // per-Gaussian frame buffers
+    tpd::EntityMats* d_ent_mats = nullptr; size_t cap_ent_mats = 0;
+    float4* d_xyzr = nullptr; size_t cap_xyzr = 0;
+    float4* d_conop = nullptr; size_t cap_conop = 0;
+    float4* d_rgb = nullptr; size_t cap_rgb = 0;
+    uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
+    uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
+    uint64_t* d_scan_desc = nullptr; size_t cap_scan_desc = 0;
+    uint32_t* d_counters = nullptr;
+    uint32_t* h_counters = nullptr;  // pinned
+
+    // pairs + sort
+    uint64_t* d_keys = nullptr; size_t cap_keys = 0;
+    uint32_t* d_vals = nullptr; size_t cap_vals = 0;
+    uint64_t* d_keys_alt = nullptr; size_t cap_keys_alt = 0;
+    uint32_t* d_vals_alt = nullptr; size_t cap_vals_alt = 0;
+    uint64_t* d_keys_dbg = nullptr; size_t cap_keys_dbg = 0;
+    uint32_t* d_vals_dbg = nullptr; size_t cap_vals_dbg = 0;
+    tpd::SortWorkspace ws;
+    const uint64_t* sorted_keys = nullptr;
+    const uint32_t* sorted_vals = nullptr;
+
+    // image
+    uint2* d_ranges = nullptr; size_t cap_ranges = 0;
+    float* d_image = nullptr; size_t cap_image = 0;
+    uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
+
+    uint32_t last_pairs = 0;
+    uint32_t last_passes = 0;
+    bool rendered = false;
+
+    // stage timing with HIP events on the render stream (tpd_set_profiling)
+    bool profile = false;
+    hipEvent_t ev[8] = {};
+
+    // backward and batched state (backward.cpp, batch.cpp)
+    struct BackwardState* bwd = nullptr;
+    struct BatchState* batch = nullptr;
+};
+
+namespace tpd {
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+uint32_t bit_length(uint32_t v);
+int ensure_scene_buffers(tpd_ctx* c);
+int ensure_image_buffers(tpd_ctx* c);
+int ensure_pair_buffers(tpd_ctx* c, size_t pairs);
+int ensure_sort_workspace(tpd_ctx* c, size_t pairs);
+int render_forward(tpd_ctx* c, hipStream_t s);
+void backward_release(tpd_ctx* c);
+void batch_release(tpd_ctx* c);
+}  // namespace tpd

@@ -1,0 +1,294 @@
+// Device-side building blocks shared by the forward, backward and batched kernels.
+//
+// Every translation unit that includes this header is compiled with -ffp-contract=off and
+// IEEE-rounded division and square root, so the float operation sequence written here is
+// exactly the one executed.  That sequence follows the reference's Slang source operation by
+// operation (files cited per function); the CPU oracle restates the same sequence, which makes
+// splats, keys, ranges and pixels bit-identical between the two.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpd {
+
+constexpr uint32_t kTile = 16;                 // BLOCK_X = BLOCK_Y (splat.slang:8-9)
+constexpr uint32_t kTilePixels = kTile * kTile;
+constexpr uint32_t kPreBlock = 256;            // Gaussians per preprocess partition (WORKGROUP_SIZE)
+constexpr uint32_t kWave = 64;
+
+// Camera uniform block (splat.slang:18-22) plus the image size the reference reads from the target.
+struct CameraParams {
+    float view[16];      // world -> view, row-major
+    float proj_view[16]; // world -> clip, row-major (projection * view, compensated on the host)
+    float focal_ndc[2];  // (P[0][0], P[1][1])
+    uint32_t width, height;
+};
+
+// Per-entity products view*model and projView*model, 32 floats per entity.
+struct EntityMats {
+    float vm[16];
+    float pvm[16];
+};
+
+__device__ __forceinline__ float as_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+__device__ __forceinline__ uint32_t as_uint(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// exp(x) for x <= 0: Cody-Waite reduction x = n ln2 + r (two-constant split of ln 2), then the
+// degree-7 Taylor polynomial of e^r in Horner form with fused multiply-adds, scaled by 2^n built
+// from the exponent bits.  Below -86 (and for NaN) it returns 0.  Identical bits on CPU and GPU
+// because it uses only correctly rounded operations.
+__device__ __forceinline__ float soft_expf(float x) {
+    if (!(x >= -86.0f)) return 0.0f;
+    if (x > 88.0f) return __builtin_inff();
+    const float n = __builtin_rintf(x * 1.44269504088896341f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
+    float p = 1.0f / 5040.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const int32_t e = (int32_t)n + 127;
+    return p * as_float((uint32_t)e << 23);
+}
+
+// Float -> UNORM8 for the RGBA8 target (blend.slang:104): clamp to [0, 1], scale by 255, round to
+// nearest even.
+__device__ __forceinline__ uint8_t unorm8(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v > 1.0f) v = 1.0f;
+    return (uint8_t)__builtin_rintf(v * 255.0f);
+}
+
+// Saturating float -> int32 truncation (NaN -> 0), written out so that the behaviour does not
+// depend on the conversion instruction.
+__device__ __forceinline__ int32_t sat_i32(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+
+__device__ __forceinline__ int32_t clamp_grid(int32_t v, uint32_t g) {
+    v = v < 0 ? 0 : v;
+    return v > (int32_t)g ? (int32_t)g : v;
+}
+
+// volume.slang:12-17 getBoundingRect.
+__device__ __forceinline__ void bounding_rect(float px, float py, float radius, uint32_t gx, uint32_t gy,
+                                              int32_t& x0, int32_t& y0, int32_t& x1, int32_t& y1) {
+    const float b = 16.0f;
+    x0 = clamp_grid(sat_i32((px - radius) / b), gx);
+    y0 = clamp_grid(sat_i32((py - radius) / b), gy);
+    x1 = clamp_grid(sat_i32((((px + radius) + b) - 1.0f) / b), gx);
+    y1 = clamp_grid(sat_i32((((py + radius) + b) - 1.0f) / b), gy);
+}
+
+__device__ __forceinline__ float row_dot_p1(const float* m, int i, float x, float y, float z) {
+    return ((m[i * 4 + 0] * x + m[i * 4 + 1] * y) + m[i * 4 + 2] * z) + m[i * 4 + 3];
+}
+
+// splat/common.slang:35-80 (band-3 "+ feature 12" typo included).  sh points at the 48-float
+// coefficient block; dir is the unit view direction.
+__device__ __forceinline__ void eval_sh(const float* sh, float x, float y, float z, uint32_t degree, float* out) {
+    const float C0 = 0.28209479177387814f;
+    const float C1 = 0.4886025119029199f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = C0 * sh[c];
+    if (degree > 0) {
+        const float a = C1 * y, b = C1 * z, d = C1 * x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            out[c] = ((out[c] - a * sh[3 + c * 15 + 0]) + b * sh[3 + c * 15 + 1]) - d * sh[3 + c * 15 + 2];
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, zx = z * x;
+            const float b4 = 1.0925484305920792f * xy, b5 = -1.0925484305920792f * yz,
+                        b6 = 0.31539156525252005f * ((2.0f * zz - xx) - yy), b7 = -1.0925484305920792f * zx,
+                        b8 = 0.5462742152960396f * (xx - yy);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float* f = sh + 3 + c * 15;
+                out[c] = ((((out[c] + b4 * f[3]) + b5 * f[4]) + b6 * f[5]) + b7 * f[6]) + b8 * f[7];
+            }
+            if (degree > 2) {
+                const float b9 = (-0.5900435899266435f * y) * (3.0f * xx - yy);
+                const float b10 = (2.890611442640554f * xy) * z;
+                const float b11 = (-0.4570457994644658f * y) * ((4.0f * zz - xx) - yy);
+                const float s12 = (0.3731763325901154f * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+                const float b13 = (-0.4570457994644658f * x) * ((4.0f * zz - xx) - yy);
+                const float b14 = (1.445305721320277f * z) * (xx - yy);
+                const float b15 = (-0.5900435899266435f * x) * (xx - 3.0f * yy);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float* f = sh + 3 + c * 15;
+                    out[c] = ((((((((out[c] + b9 * f[8]) + b10 * f[9]) + b11 * f[10]) + s12) + f[11]) + b13 * f[12])
+                               + b14 * f[13]) + b15 * f[14]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float v = out[c] + 0.5f;
+        out[c] = v > 0.0f ? v : 0.0f;
+    }
+}
+
+struct Projected {
+    float px, py, vz, radius;
+    float con_a, con_b, con_c, opacity;
+    float r, g, b;
+    uint32_t tiles;
+    int32_t x0, y0, x1, y1;
+};
+
+// project.slang:33-90 for the Gaussian record g (60 floats: position, opacity, quaternion xyzw,
+// scale xyz + modifier, sh[48]).  Returns false for culled / degenerate / tile-less Gaussians,
+// in which case p.tiles = 0 and p.radius = 0.
+__device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, const float* vm, const float* pvm,
+                                                 const float* V, float focal_x, float focal_y, uint32_t W, uint32_t H,
+                                                 uint32_t gx, uint32_t gy, uint32_t sh_degree, Projected& p) {
+    p.tiles = 0;
+    p.radius = 0.0f;
+    const float mx = g[0], my = g[1], mz = g[2];
+    // splat/common.slang:98-119 passFrustumClipping
+    const float vx = row_dot_p1(vm, 0, mx, my, mz), vy = row_dot_p1(vm, 1, mx, my, mz),
+                vz = row_dot_p1(vm, 2, mx, my, mz);
+    if (vz <= 0.0f) return false;
+    const float cx = row_dot_p1(pvm, 0, mx, my, mz), cy = row_dot_p1(pvm, 1, mx, my, mz),
+                cz = row_dot_p1(pvm, 2, mx, my, mz), cw = row_dot_p1(pvm, 3, mx, my, mz);
+    if (cx < -1.3f * cw || cx > 1.3f * cw) return false;
+    if (cy < -1.3f * cw || cy > 1.3f * cw) return false;
+    if (cz < 0.0f || cz > cw) return false;
+    const float inv_w = 1.0f / cw;
+    const float ndc_x = cx * inv_w, ndc_y = cy * inv_w;
+
+    // volume.slang:20-41 computeCovariance
+    const float sx = g[8] * g[11], sy = g[9] * g[11], sz = g[10] * g[11];
+    const float qx = g[4], qy = g[5], qz = g[6], qw = g[7];
+    const float R00 = 1.0f - 2.0f * (qy * qy + qz * qz), R01 = 2.0f * (qx * qy - qw * qz),
+                R02 = 2.0f * (qx * qz + qw * qy);
+    const float R10 = 2.0f * (qx * qy + qw * qz), R11 = 1.0f - 2.0f * (qx * qx + qz * qz),
+                R12 = 2.0f * (qy * qz - qw * qx);
+    const float R20 = 2.0f * (qx * qz - qw * qy), R21 = 2.0f * (qy * qz + qw * qx),
+                R22 = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float M00 = R00 * sx, M01 = R01 * sy, M02 = R02 * sz;
+    const float M10 = R10 * sx, M11 = R11 * sy, M12 = R12 * sz;
+    const float M20 = R20 * sx, M21 = R21 * sy, M22 = R22 * sz;
+    const float C00 = (M00 * M00 + M01 * M01) + M02 * M02;
+    const float C01 = (M00 * M10 + M01 * M11) + M02 * M12;
+    const float C02 = (M00 * M20 + M01 * M21) + M02 * M22;
+    const float C10 = (M10 * M00 + M11 * M01) + M12 * M02;
+    const float C11 = (M10 * M10 + M11 * M11) + M12 * M12;
+    const float C12 = (M10 * M20 + M11 * M21) + M12 * M22;
+    const float C20 = (M20 * M00 + M21 * M01) + M22 * M02;
+    const float C21 = (M20 * M10 + M21 * M11) + M22 * M12;
+    const float C22 = (M20 * M20 + M21 * M21) + M22 * M22;
+
+    // volume.slang:44-63 projectCovariance + computeJacobian (no clamp of x/z, y/z)
+    const float fxz = focal_x / vz, fyz = focal_y / vz;
+    const float tx = vx / vz, ty = vy / vz;
+    const float j02 = -fxz * tx, j12 = -fyz * ty;
+    const float T00 = fxz * V[0] + j02 * V[8], T01 = fxz * V[1] + j02 * V[9], T02 = fxz * V[2] + j02 * V[10];
+    const float T10 = fyz * V[4] + j12 * V[8], T11 = fyz * V[5] + j12 * V[9], T12 = fyz * V[6] + j12 * V[10];
+    // (C T^T)[k][s]
+    const float A00 = (C00 * T00 + C01 * T01) + C02 * T02, A01 = (C00 * T10 + C01 * T11) + C02 * T12;
+    const float A10 = (C10 * T00 + C11 * T01) + C12 * T02, A11 = (C10 * T10 + C11 * T11) + C12 * T12;
+    const float A20 = (C20 * T00 + C21 * T01) + C22 * T02, A21 = (C20 * T10 + C21 * T11) + C22 * T12;
+    const float c00 = (T00 * A00 + T01 * A10) + T02 * A20;
+    const float c10 = (T10 * A00 + T11 * A10) + T12 * A20;
+    const float c11 = (T10 * A01 + T11 * A11) + T12 * A21;
+
+    // project.slang:57-70
+    const float ca = c00 + 0.3f, cb = c10, cc = c11 + 0.3f;
+    const float det = ca * cc - cb * cb;
+    if (det == 0.0f) return false;
+    const float det_inv = 1.0f / det;
+    const float mid = 0.5f * (ca + cc);
+    float disc = mid * mid - det;
+    disc = disc > 0.1f ? disc : 0.1f;
+    const float sq = __builtin_sqrtf(disc);
+    const float l1 = mid + sq, l2 = mid - sq;
+    const float radius = __builtin_ceilf(3.0f * __builtin_sqrtf(l1 > l2 ? l1 : l2));
+
+    // volume.slang:3-8 ndc2pix, then the tile rectangle
+    const float px = ((ndc_x + 1.0f) * (float)W - 1.0f) * 0.5f;
+    const float py = ((ndc_y + 1.0f) * (float)H - 1.0f) * 0.5f;
+    int32_t x0, y0, x1, y1;
+    bounding_rect(px, py, radius, gx, gy, x0, y0, x1, y1);
+    const uint32_t tiles = (uint32_t)(x1 - x0) * (uint32_t)(y1 - y0);
+    if (tiles == 0) return false;
+
+    // common.slang:88-92 camera position, colour from the untransformed mean (project.slang:75-76)
+    const float camx = -((V[0] * V[3] + V[4] * V[7]) + V[8] * V[11]);
+    const float camy = -((V[1] * V[3] + V[5] * V[7]) + V[9] * V[11]);
+    const float camz = -((V[2] * V[3] + V[6] * V[7]) + V[10] * V[11]);
+    const float dx = mx - camx, dy = my - camy, dz = mz - camz;
+    const float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
+    const float inv_len = 1.0f / len;
+    float col[3];
+    eval_sh(g + 12, dx * inv_len, dy * inv_len, dz * inv_len, sh_degree, col);
+
+    p.px = px; p.py = py; p.vz = vz; p.radius = radius;
+    p.con_a = cc * det_inv; p.con_b = -cb * det_inv; p.con_c = ca * det_inv; p.opacity = g[3];
+    p.r = col[0]; p.g = col[1]; p.b = col[2];
+    p.tiles = tiles;
+    p.x0 = x0; p.y0 = y0; p.x1 = x1; p.y1 = y1;
+    return true;
+}
+
+// ------------------------------------------------------------------ wave64 helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t l = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (l >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Relaxed device-scope atomic load/store used by the decoupled look-back descriptors.
+template <typename T>
+__device__ __forceinline__ T load_relaxed(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void store_relaxed(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Logical block index remapped so that each of the 8 XCDs (which receive workgroups round-robin)
+// works on one contiguous range of the index space; neighbouring tiles then share an L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u;
+    const uint32_t xcd = b % 8u, pos = b / 8u;
+    return xcd < r ? xcd * (q + 1u) + pos : r * (q + 1u) + (xcd - r) * q + pos;
+}
+
+}  // namespace tpd

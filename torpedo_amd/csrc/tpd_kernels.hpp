@@ -1,0 +1,84 @@
+// Kernel argument blocks and launchers shared by the kernels and the C-ABI layer.
+#pragma once
+
+#include "tpd_device.hpp"
+
+namespace tpd {
+
+// Indices into the per-context device counter block.
+enum : uint32_t {
+    kCtrScanTicket = 0,   // preprocess partition ticket
+    kCtrPairs = 1,        // pair count P written by the last preprocess partition
+    kCtrOverflow = 2,     // 1 when P exceeded the key/value capacity
+    kCtrCount = 16,
+};
+
+struct ForwardArgs {
+    const float* gauss;          // N x 60 floats (GaussianPoint records)
+    const uint32_t* ent_idx;     // N entity indices, or null (entity 0 for all)
+    const EntityMats* ent;       // per-entity matrices
+    CameraParams cam;
+    uint32_t n, sh_degree, gx, gy;
+    uint32_t cap;                // pair capacity of keys/vals
+    float4* xyzr;                // image x, image y, view depth, radius
+    float4* conop;               // conic a, b, c, opacity
+    float4* rgb;                 // colour rgb, 0
+    uint32_t* tiles;             // tile count per Gaussian
+    uint32_t* offsets;           // exclusive scan of tiles
+    uint64_t* scan_desc;         // ceil(N/256) look-back descriptors, zero on entry
+    uint32_t* counters;          // kCtr* block
+    uint64_t* keys;
+    uint32_t* vals;
+    uint32_t val_base;           // value written for Gaussian i is val_base + i
+    uint2* ranges;               // zeroed here for the ranges kernel
+    uint32_t ntiles;
+};
+
+struct BlendArgs {
+    const uint2* ranges;
+    const uint32_t* vals;
+    const float4* xyzr;
+    const float4* conop;
+    const float4* rgb;
+    uint32_t width, height, gx, ntiles;
+    float* image;            // H x W x 3 or null
+    uint8_t* rgba8;          // H x W x 4 or null
+};
+
+void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
+void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
+void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
+                   uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s);
+void launch_blend(const BlendArgs& a, hipStream_t s);
+
+// ------------------------------------------------------------------ radix sort
+
+constexpr uint32_t kRadixBits = 8;
+constexpr uint32_t kRadix = 1u << kRadixBits;
+constexpr uint32_t kSortBlock = 256;
+constexpr uint32_t kSortItems = 16;
+constexpr uint32_t kSortTile = kSortBlock * kSortItems;  // keys per onesweep partition
+constexpr uint32_t kMaxPasses = 8;
+constexpr uint32_t kHistBlocks = 512;                      // blocks of the digit-histogram kernel
+constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back values
+
+struct SortWorkspace {
+    uint32_t* desc = nullptr;      // 2 regions x parts_cap x kRadix look-back descriptors
+    uint32_t parts_cap = 0;
+    uint32_t* partials = nullptr;  // kHistBlocks x kMaxPasses x kRadix block histograms
+    uint32_t* offsets = nullptr;   // kMaxPasses x kRadix global exclusive digit offsets
+    uint32_t* tickets = nullptr;   // kMaxPasses partition tickets
+};
+
+__host__ __device__ inline uint32_t sort_parts(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+__host__ __device__ inline uint32_t sort_passes(uint32_t key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
+
+// Stable LSD sort of (keys, vals) by the low key_bits bits of the keys.  n comes from *count_ptr
+// when count_ptr is non-null (device-resident count, grids sized by n_max), else from n_max.
+// Returns true when the sorted data ended in the alternate buffers.
+// after_hist (optional) is recorded between the histogram kernels and the first onesweep pass.
+bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                       const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
+                       hipEvent_t after_hist = nullptr);
+
+}  // namespace tpd

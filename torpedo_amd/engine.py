@@ -1,0 +1,366 @@
+"""Python mirror of torpedo's engine interface over the C ABI (libtpd_hip.so).
+
+The reference's interface (C++) and its Python counterpart here:
+
+    tpd::Scene::add(ent::group(points)) / add(point)   Scene.add(points) / Scene.add(point)   (Scene.h:19-62)
+    tpd::GaussianEngine::compile(scene, settings)      GaussianEngine.compile(scene, settings) (GaussianEngine.cpp:359-399)
+    tpd::GaussianEngine::rasterFrame(camera)           GaussianEngine.rasterFrame(camera)      (GaussianEngine.cpp:621-712)
+    tpd::GaussianEngine::getTransformHost()            GaussianEngine.getTransformHost()       (GaussianEngine.h:24)
+    tpd::TransformHost::transform(entity, mat4)        TransformHost.transform(entity, m)      (TransformHost.cpp:3-12)
+    tpd::Camera::lookAt(eye, center, up)               Camera.lookAt(eye, center, up)          (Camera.cpp:3-16)
+    tpd::PerspectiveCamera(w, h), setVerticalFov(deg)  PerspectiveCamera(w, h), setVerticalFov (PerspectiveCamera.{h,cpp})
+    tpd::GaussianPoint / utils::rgb2sh / sh2rgb        records of 60 floats, rgb2sh / sh2rgb   (GaussianGeometry.h:10-52)
+
+Error behaviour follows the reference: compiling an empty scene warns and keeps going
+(GaussianEngine.cpp:362-365), an SH degree above 3 is clamped with a warning (:366-370),
+transforming an unknown entity is ignored (TransformHost.cpp:4-6).  Failures of the device
+layer raise TpdError.  The reference has no readback; this engine adds readImage/readRGBA8 and
+the stage readbacks used by the parity tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, f32, lib, ptr
+
+SH_C0 = 0.28209479177387814
+GAUSSIAN_FLOATS = 60
+
+
+def rgb2sh(r: float, g: float, b: float) -> np.ndarray:
+    """utils::rgb2sh (GaussianGeometry.h:38-45): DC coefficients for an RGB colour, rest zero."""
+    sh = np.zeros(48, np.float32)
+    c0 = np.float32(SH_C0)
+    sh[0] = (np.float32(r) - np.float32(0.5)) / c0
+    sh[1] = (np.float32(g) - np.float32(0.5)) / c0
+    sh[2] = (np.float32(b) - np.float32(0.5)) / c0
+    return sh
+
+
+def sh2rgb(sh) -> np.ndarray:
+    """utils::sh2rgb (GaussianGeometry.h:47-52)."""
+    sh = f32(sh)
+    return sh[:3] * np.float32(SH_C0) + np.float32(0.5)
+
+
+def gaussian_point(position, opacity, quaternion=(0.0, 0.0, 0.0, 1.0), scale=(1.0, 1.0, 1.0, 1.0), sh=None):
+    """One GaussianPoint record (GaussianGeometry.h:10-30): 60 float32."""
+    rec = np.zeros(GAUSSIAN_FLOATS, np.float32)
+    rec[0:3] = position
+    rec[3] = opacity
+    rec[4:8] = quaternion
+    sc = list(scale)
+    rec[8:12] = sc + [1.0] * (4 - len(sc))
+    rec[12:60] = np.zeros(48, np.float32) if sh is None else sh
+    return rec
+
+
+class Entity(int):
+    """Handle returned by Scene.add (an entt::entity in the reference)."""
+
+
+class Scene:
+    """Scene (Scene.h:17-54): groups of points and single points; dataAll() lists groups first, then singles
+    (Scene.h:130-146), which is also the entity index order of buildEntityMap (Scene.h:154-162)."""
+
+    def __init__(self):
+        self._next = 0
+        self._groups: list[tuple[Entity, np.ndarray]] = []
+        self._singles: list[tuple[Entity, np.ndarray]] = []
+
+    def add(self, points) -> Entity:
+        a = f32(points)
+        e = Entity(self._next)
+        self._next += 1
+        if a.ndim == 1:
+            if a.size != GAUSSIAN_FLOATS:
+                raise ValueError("a single GaussianPoint has 60 floats")
+            self._singles.append((e, a.copy()))
+        else:
+            if a.ndim != 2 or a.shape[1] != GAUSSIAN_FLOATS:
+                raise ValueError("a group of GaussianPoints is an (N, 60) array")
+            self._groups.append((e, a))  # borrowed, as the reference's std::span
+        return e
+
+    def count(self) -> int:
+        return len(self._singles)
+
+    def countGroup(self) -> int:
+        return len(self._groups)
+
+    def countAll(self) -> int:
+        return sum(g.shape[0] for _, g in self._groups) + len(self._singles)
+
+    def groupSizes(self) -> list[int]:
+        return [g.shape[0] for _, g in self._groups]
+
+    def dataAll(self) -> np.ndarray:
+        parts = [g for _, g in self._groups] + [s[None, :] for _, s in self._singles]
+        if not parts:
+            return np.zeros((0, GAUSSIAN_FLOATS), np.float32)
+        return np.ascontiguousarray(np.concatenate(parts, 0), np.float32)
+
+    def buildEntityMap(self) -> dict:
+        ents = [e for e, _ in self._groups] + [e for e, _ in self._singles]
+        return {e: i for i, e in enumerate(ents)}
+
+    def entityIndices(self) -> np.ndarray:
+        """Per-Gaussian entity index (GaussianEngine.cpp:383-388)."""
+        idx = []
+        k = 0
+        for size in self.groupSizes():
+            idx.append(np.full(size, k, np.uint32))
+            k += 1
+        for _ in self._singles:
+            idx.append(np.full(1, k, np.uint32))
+            k += 1
+        return np.concatenate(idx) if idx else np.zeros(0, np.uint32)
+
+
+class Camera:
+    """Camera (Camera.h:11-38): a world-to-view matrix plus near/far planes."""
+
+    def __init__(self, w2c=None):
+        self._view = np.eye(4, dtype=np.float32) if w2c is None else f32(w2c).reshape(4, 4).copy()
+        self._near = 0.01
+        self._far = 100.0
+
+    def lookAt(self, eye, center=None, up=None):
+        """lookAt(eye, center, up) (Camera.cpp:3-16) or lookAt(R, t) (Camera.cpp:18-20)."""
+        if up is None:  # (R 3x3, t)
+            R = f32(eye).reshape(3, 3)
+            t = f32(center).reshape(3)
+            v = np.eye(4, dtype=np.float32)
+            v[:3, :3] = R
+            v[:3, 3] = t
+            self._view = v
+            return
+        v = np.zeros(16, np.float32)
+        e, c, u = f32(eye).reshape(3), f32(center).reshape(3), f32(up).reshape(3)
+        lib().tpd_look_at(ptr(e), ptr(c), ptr(u), ptr(v))
+        self._view = v.reshape(4, 4)
+
+    def setNear(self, near: float):
+        self._near = float(near)
+
+    def setFar(self, far: float):
+        self._far = float(far)
+
+    def getViewMatrix(self) -> np.ndarray:
+        return self._view.copy()
+
+    def getProjection(self) -> np.ndarray:
+        raise NotImplementedError
+
+
+class PerspectiveCamera(Camera):
+    """PerspectiveCamera (PerspectiveCamera.h:10-31): reversed-z projection, default fy = sqrt(3) (60 deg)."""
+
+    def __init__(self, width: int, height: int):
+        super().__init__()
+        self.width = int(width)
+        self.height = int(height)
+        self._fy = float(np.sqrt(np.float32(3.0)))
+
+    def setVerticalFov(self, degrees: float):
+        self._fy = float(lib().tpd_fy_from_fov(float(degrees)))
+
+    def onImageSizeChange(self, w: int, h: int):
+        self.width, self.height = int(w), int(h)
+
+    def getProjection(self) -> np.ndarray:
+        p = np.zeros(16, np.float32)
+        lib().tpd_perspective(self.width, self.height, self._fy, self._near, self._far, ptr(p))
+        return p.reshape(4, 4)
+
+    def uniforms(self):
+        """(view, projView, focalNDC) as uploaded by updateCameraBuffer (GaussianEngine.cpp:764-775)."""
+        view = f32(self._view).ravel()
+        proj = f32(self.getProjection()).ravel()
+        pv = np.zeros(16, np.float32)
+        fo = np.zeros(2, np.float32)
+        lib().tpd_camera_upload(ptr(view), ptr(proj), ptr(pv), ptr(fo))
+        return view.reshape(4, 4), pv.reshape(4, 4), fo
+
+
+@dataclass
+class Settings:
+    """GaussianEngine::Settings (GaussianEngine.h:17-21)."""
+    sphericalHarmonicsDegree: int = 3
+
+    @staticmethod
+    def getDefault() -> "Settings":
+        return Settings()
+
+
+class TransformHost:
+    """TransformHost (TransformHost.h): per-entity model matrices."""
+
+    def __init__(self, engine: "GaussianEngine", entity_map: dict):
+        self._engine = engine
+        self._map = dict(entity_map)
+
+    def transform(self, entity, matrix):
+        if entity not in self._map:
+            return  # TransformHost.cpp:4-6
+        m = f32(matrix).reshape(16)
+        check(lib().tpd_set_transform(self._engine._ctx, self._map[entity], ptr(m)))
+
+
+class GaussianEngine:
+    """GaussianEngine (GaussianEngine.h:15-27) over libtpd_hip.so, headless."""
+
+    def __init__(self, device: int = 0, keep_unsorted: bool = False):
+        cfg = _capi.TpdConfig()
+        cfg.device = device
+        cfg.keep_unsorted = 1 if keep_unsorted else 0
+        h = ctypes.c_void_p()
+        check(lib().tpd_create(ctypes.byref(h), ctypes.byref(cfg)))
+        self._ctx = h
+        self._n = 0
+        self._w = self._h = 0
+        self._transform_host = None
+        self._scene_ref = None
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib().tpd_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- compile -------------------------------------------------------------------------------
+    def compile(self, scene: Scene, settings: Settings | None = None):
+        settings = settings or Settings.getDefault()
+        n = scene.countAll()
+        if n == 0:
+            warnings.warn("GaussianEngine - Scene compilation warning: Could NOT find a single GaussianPoint in the scene!")
+        deg = int(settings.sphericalHarmonicsDegree)
+        if deg > 3:
+            warnings.warn("GaussianEngine - Clamping to the maximum degrees of SH support: up to 3 degrees only")
+            deg = 3
+        data = scene.dataAll()
+        emap = scene.buildEntityMap()
+        n_ent = max(len(emap), 1)
+        idx = scene.entityIndices() if len(emap) > 1 else None
+        models = np.tile(np.eye(4, dtype=np.float32).reshape(1, 16), (n_ent, 1))
+        check(lib().tpd_upload(self._ctx, ptr(data) if n else None, n, ptr(idx), ptr(models) if idx is not None else None,
+                               n_ent if idx is not None else 0, deg))
+        self._n = n
+        self._transform_host = TransformHost(self, emap)
+
+    def uploadRecords(self, records, sh_degree: int = 3):
+        """compile() for an (N, 60) record array with a single identity entity."""
+        data = f32(records).reshape(-1, GAUSSIAN_FLOATS)
+        n = data.shape[0]
+        check(lib().tpd_upload(self._ctx, ptr(data) if n else None, n, None, None, 0, min(int(sh_degree), 3)))
+        self._n = n
+        self._transform_host = TransformHost(self, {Entity(0): 0})
+
+    def getTransformHost(self) -> TransformHost:
+        return self._transform_host
+
+    # -- frame ---------------------------------------------------------------------------------
+    def setCamera(self, view, proj_view, focal_ndc, width: int, height: int):
+        v, pv, fo = f32(view).reshape(16), f32(proj_view).reshape(16), f32(focal_ndc).reshape(2)
+        check(lib().tpd_set_camera(self._ctx, ptr(v), ptr(pv), ptr(fo), int(width), int(height)))
+        self._w, self._h = int(width), int(height)
+
+    def rasterFrame(self, camera: PerspectiveCamera, stream=None):
+        view, pv, fo = camera.uniforms()
+        self.setCamera(view, pv, fo, camera.width, camera.height)
+        self.render(stream)
+
+    def render(self, stream=None):
+        check(lib().tpd_render(self._ctx, stream))
+
+    def waitIdle(self):
+        """Engine::waitIdle (Engine.h:19)."""
+        check(lib().tpd_synchronize(self._ctx))
+
+    # -- readbacks -----------------------------------------------------------------------------
+    @property
+    def pairCount(self) -> int:
+        v = ctypes.c_uint32()
+        check(lib().tpd_get_pair_count(self._ctx, ctypes.byref(v)))
+        return int(v.value)
+
+    def readImage(self) -> np.ndarray:
+        img = np.empty((self._h, self._w, 3), np.float32)
+        check(lib().tpd_read_image_f32(self._ctx, ptr(img)))
+        return img
+
+    def readRGBA8(self) -> np.ndarray:
+        img = np.empty((self._h, self._w, 4), np.uint8)
+        check(lib().tpd_read_image_rgba8(self._ctx, ptr(img)))
+        return img
+
+    def readSplats(self) -> dict:
+        raw = np.empty((max(self._n, 1), 12), np.float32)
+        check(lib().tpd_read_splats(self._ctx, ptr(raw)))
+        raw = raw[: self._n]
+        return dict(color=raw[:, 0:3].copy(), tiles=raw[:, 3].copy().view(np.uint32), texel=raw[:, 4:8].copy(),
+                    copac=raw[:, 8:12].copy())
+
+    def readTiles(self):
+        counts = np.empty(max(self._n, 1), np.uint32)
+        offs = np.empty(max(self._n, 1), np.uint32)
+        check(lib().tpd_read_tiles(self._ctx, ptr(counts), ptr(offs)))
+        return counts[: self._n], offs[: self._n]
+
+    def readPairs(self, unsorted: bool = False):
+        P = self.pairCount
+        k = np.empty(max(P, 1), np.uint64)
+        v = np.empty(max(P, 1), np.uint32)
+        check(lib().tpd_read_pairs(self._ctx, ptr(k), ptr(v), max(P, 1), 1 if unsorted else 0))
+        return k[:P], v[:P]
+
+    def readRanges(self) -> np.ndarray:
+        gx, gy = (self._w + 15) // 16, (self._h + 15) // 16
+        r = np.empty((gx * gy, 2), np.uint32)
+        check(lib().tpd_read_ranges(self._ctx, ptr(r)))
+        return r
+
+    def setProfiling(self, enable: bool = True):
+        check(lib().tpd_set_profiling(self._ctx, 1 if enable else 0))
+
+    STAGES = ("preprocess_scan_duplicate", "sort_histogram", "sort_passes", "ranges", "blend", "frame")
+
+    def stageTimes(self) -> dict:
+        """Milliseconds per stage of the last profiled frame (HIP events on the render stream)."""
+        ms = np.zeros(6, np.float32)
+        passes = ctypes.c_uint32()
+        check(lib().tpd_get_stage_times(self._ctx, ptr(ms), ctypes.byref(passes)))
+        out = {k: float(v) for k, v in zip(self.STAGES, ms)}
+        out["radix_passes"] = int(passes.value)
+        return out
+
+    def deviceImagePtr(self) -> int:
+        p = ctypes.c_void_p()
+        check(lib().tpd_device_image(self._ctx, ctypes.byref(p)))
+        return int(p.value or 0)
+
+
+def sort_pairs_device(engine: GaussianEngine, keys_ptr: int, vals_ptr: int, keys_alt_ptr: int, vals_alt_ptr: int,
+                      n: int, key_bits: int, stream: int | None = None) -> bool:
+    """tpd_sort_pairs on device buffers (e.g. torch tensors' data_ptr()); returns True if the result is in *_alt."""
+    in_alt = ctypes.c_int()
+    check(lib().tpd_sort_pairs(engine._ctx, ctypes.c_void_p(keys_ptr), ctypes.c_void_p(vals_ptr),
+                               ctypes.c_void_p(keys_alt_ptr), ctypes.c_void_p(vals_alt_ptr), int(n), int(key_bits),
+                               None if stream is None else ctypes.c_void_p(stream), ctypes.byref(in_alt)))
+    return bool(in_alt.value)
