@@ -40,11 +40,15 @@ def camera_uniforms(eye, w, h):
     return view, pv, fo
 
 
-def time_plain(eng, g, eye, w, h, steps, warmup):
+def time_plain(eng, g, eye, w, h, steps, warmup, tune=False):
     eng.uploadRecords(g, 3)
     view, pv, fo = camera_uniforms(eye, w, h)
     eng.setCamera(view, pv, fo, w, h)
     eng.setProfiling(True)
+    tuned = None
+    if tune:
+        eng.render()
+        tuned = eng.autotune()
     for _ in range(warmup):
         eng.render()
     eng.waitIdle()
@@ -57,6 +61,8 @@ def time_plain(eng, g, eye, w, h, steps, warmup):
     dt = time.perf_counter() - t0
     P = eng.pairCount
     med = {k: float(np.median([s[k] for s in stages])) for k in stages[0]}
+    if tuned is not None:
+        med["autotune"] = tuned
     return dt, P, med
 
 
@@ -80,6 +86,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sort-shape", type=int, default=-1,
+                    help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
+    ap.add_argument("--only-sort-load", action="store_true",
+                    help="run only the sort-load frames (for profiler runs of the radix pass)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); views and scenes would shard "
@@ -91,17 +101,32 @@ def main():
 
     W, H = 1920, 1080
     eye = [0.0, 0.0, -8.0]
+    tune = args.sort_shape < 0
+    eng = GaussianEngine(device=0)
+    if not tune:
+        eng.setSortShape(args.sort_shape)
+    if args.only_sort_load:
+        g2 = scenes.cluster_scene(1_000_000, seed=2, log_scale_mean=-3.0)
+        dt2, P2, st2 = time_plain(eng, g2, eye, W, H, args.steps, args.warmup, tune=tune)
+        t_launch = st2["sort_passes"] / 1000.0 / st2["radix_passes"]
+        # A plain copy of 256 MiB (read + write = 512 MiB per launch) for counter calibration.
+        copy_gbs = eng.measureCopy(256 << 20, 3)
+        print(json.dumps({"workload": "clusters-1M-1920x1080 log-scale mean -3", "pairs": P2, "stage_ms": st2,
+                          "onesweep_avg_launch_us": round(t_launch * 1e6, 2),
+                          "onesweep_gbs": round(24.0 * P2 / t_launch / 1e9, 1),
+                          "copy_256MiB_gbs": round(copy_gbs, 1)}), flush=True)
+        eng.close()
+        return
     log("[bench] generating clusters-1M scene")
     g = scenes.cluster_scene(1_000_000, seed=2)
-    eng = GaussianEngine(device=0)
     log("[bench] timing plain forward frames")
-    dt, P, st = time_plain(eng, g, eye, W, H, args.steps, args.warmup)
+    dt, P, st = time_plain(eng, g, eye, W, H, args.steps, args.warmup, tune=tune)
     ms = 1000.0 * dt / args.steps
     log(f"[bench] plain: {ms:.3f} ms/frame, P={P}, stages {st}")
 
     log("[bench] sort-load variant (log-scale mean -3)")
     g2 = scenes.cluster_scene(1_000_000, seed=2, log_scale_mean=-3.0)
-    dt2, P2, st2 = time_plain(eng, g2, eye, W, H, max(5, args.steps // 2), 2)
+    dt2, P2, st2 = time_plain(eng, g2, eye, W, H, max(5, args.steps // 2), 2, tune=tune)
     log(f"[bench] sort-load: P={P2}, stages {st2}")
 
     # Roofline of the radix pass (onesweep) kernel: 24 algorithmic bytes per pair per pass
@@ -115,6 +140,14 @@ def main():
                 "avg_launch_us": round(t_launch_s * 1e6, 2)}
     t_launch_c2 = st["sort_passes"] / 1000.0 / max(st["radix_passes"], 1)
     sort_frac_c2 = 24.0 * P / t_launch_c2 / 1e9 / HBM_PEAK_GBS if P else 0.0
+    copy_gbs = eng.measureCopy(1 << 30, 10)
+    log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
+    # Blend: ~22 FP32 operations + 1 exp per evaluated pixel-splat pair (device-counted).
+    ev = st["blend_evaluations"]
+    blend_tflops = 22.0 * ev / (st["blend"] / 1000.0) / 1e12
+    roofline_blend = {"bound": "valu", "achieved": round(blend_tflops, 3), "peak": 157.3, "unit": "TFLOP/s",
+                      "frac": round(blend_tflops / 157.3, 4), "evaluations": ev, "kernel": "k_blend",
+                      "note": "22 flop per evaluated pixel-splat pair (plus one exp), vector FP32 peak"}
 
     out = {
         "metric": METRIC,
@@ -132,10 +165,14 @@ def main():
         "config": {"workload": "clusters-1M-1920x1080 plain forward", "gaussians": 1_000_000, "width": W,
                    "height": H, "sh_degree": 3, "tile": 16, "pairs": P, "sort_digit_bits": 8,
                    "parallelism": "single GPU"},
-        "stage_ms": {k: round(v, 4) for k, v in st.items() if k != "radix_passes"},
+        "stage_ms": {k: round(v, 4) for k, v in st.items()
+                     if k not in ("radix_passes", "autotune", "blend_evaluations")},
+        "autotune": {"clusters_1M": st.get("autotune"), "sort_load_variant": st2.get("autotune")},
         "sort_hbm_frac": {"sort_load_variant": round(achieved / HBM_PEAK_GBS, 4), "pairs": P2,
                           "clusters_1M": round(sort_frac_c2, 4), "pairs_clusters_1M": P},
         "roofline": roofline,
+        "roofline_blend": roofline_blend,
+        "copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle restatement, one frame)")

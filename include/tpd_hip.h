@@ -105,12 +105,24 @@ int tpd_device_image(tpd_ctx* ctx, float** d_rgb);
  * blend and whole-frame times in milliseconds; *passes the number of radix passes. */
 int tpd_set_profiling(tpd_ctx* ctx, int enable);
 int tpd_get_stage_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
+/* Pixel-splat pairs whose power the blend evaluated in the last profiled frame. */
+int tpd_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
+/* Time of one plain device-to-device copy of `bytes` (bandwidth calibration), averaged over iters. */
+int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per_copy);
 
 /* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----
  * Sorts n (key, value) pairs by key (stable); key bits at and above key_bits must be zero.  d_keys_alt /
  * d_vals_alt are scratch of the same size; *in_alt is set to 1 when the result is there. */
 int tpd_sort_pairs(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_keys_alt, uint32_t* d_vals_alt,
                    uint32_t n, uint32_t key_bits, void* stream, int* in_alt);
+
+/* ----- autotuning (the reference's shapes are compile-time constants, GaussianEngine.h:121-124) -----
+ * Times every radix-pass block shape on the last rendered frame's pairs and keeps the fastest
+ * (results are identical for every shape).  sort_ms[0..capacity) receives the median time of a
+ * full sort per shape; *n_shapes the number of shapes; *best the chosen one.  Re-renders the frame. */
+int tpd_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
+/* Selects a radix-pass block shape by index (0 = default). */
+int tpd_set_sort_shape(tpd_ctx* ctx, uint32_t shape);
 
 /* ----- host camera math (torpedo/math, rendering/src/Camera.cpp, extension/src/PerspectiveCamera.cpp) ----- */
 /* Camera::lookAt (Camera.cpp:3-16) */

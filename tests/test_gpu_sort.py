@@ -67,3 +67,22 @@ def test_sort_empty_and_invalid(eng):
     assert k.size == 0
     with pytest.raises(TpdError):
         gpu_sort(eng, np.zeros(4, np.uint64), np.zeros(4, np.uint32), 0)
+
+
+@pytest.mark.parametrize("shape", range(6))
+def test_every_block_shape(eng, shape):
+    print(f"[sort] shape={shape}", flush=True)
+    eng.setSortShape(shape)
+    try:
+        rng = np.random.default_rng(100 + shape)
+        for n in (3, 2047, 2049, 8191, 12289, 700_001):
+            tiles = rng.integers(0, 3600, n, dtype=np.uint64)
+            depth = rng.uniform(0.01, 100.0, n).astype(np.float32).view(np.uint32).astype(np.uint64)
+            keys = (tiles << np.uint64(32)) | depth
+            keys[: n // 3] = keys[0]  # a long run of one key
+            vals = np.arange(n, dtype=np.uint32)
+            k, v = gpu_sort(eng, keys, vals, 44)
+            o = np.argsort(keys, kind="stable")
+            assert np.array_equal(k, keys[o]) and np.array_equal(v, vals[o]), (shape, n)
+    finally:
+        eng.setSortShape(0)

@@ -49,7 +49,11 @@ SIGNATURES = {
     "tpd_device_image": (_i32, [_vp, ctypes.POINTER(_vp)]),
     "tpd_set_profiling": (_i32, [_vp, _i32]),
     "tpd_get_stage_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
+    "tpd_get_blend_evaluations": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "tpd_measure_copy": (_i32, [_vp, ctypes.c_uint64, _u32, ctypes.POINTER(_f32)]),
     "tpd_sort_pairs": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(_i32)]),
+    "tpd_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "tpd_set_sort_shape": (_i32, [_vp, _u32]),
     "tpd_look_at": (None, [_vp, _vp, _vp, _vp]),
     "tpd_perspective": (None, [_u32, _u32, _f32, _f32, _f32, _vp]),
     "tpd_fy_from_fov": (_f32, [_f32]),

@@ -117,8 +117,9 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs) {
     const uint32_t parts = sort_parts((uint32_t)std::min<size_t>(pairs, kMaxSortKeys));
     hipError_t e;
     if (!c->ws.partials) {
-        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kHistBlocks * kMaxPasses * kRadix)) != hipSuccess)
-            return hip_fail(e, "alloc sort partials");
+        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess ||
+            (e = hipMemset(c->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess)
+            return hip_fail(e, "alloc sort counts");
         if ((e = hipMalloc(&c->ws.offsets, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort offsets");
         if ((e = hipMalloc(&c->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
@@ -214,6 +215,7 @@ int tpd_destroy(tpd_ctx* c) {
     release(c->d_ranges);
     release(c->d_image);
     release(c->d_rgba8);
+    release(c->d_eval_count);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -436,6 +438,13 @@ int tpd_get_stage_times(tpd_ctx* c, float* ms, uint32_t* passes) {
     }
     if (passes) *passes = c->last_passes;
     return TPD_OK;
+}
+
+int tpd_get_blend_evaluations(tpd_ctx* c, uint64_t* evaluations) {
+    if (!check_ctx(c) || !evaluations) return fail(TPD_ERR_INVALID, "tpd_get_blend_evaluations: bad argument");
+    if (!c->profile || !c->rendered || !c->d_eval_count)
+        return fail(TPD_ERR_STATE, "tpd_get_blend_evaluations: no profiled frame");
+    return copy_back(c, evaluations, c->d_eval_count, sizeof(uint64_t));
 }
 
 int tpd_device_image(tpd_ctx* c, float** d_rgb) {

@@ -228,65 +228,124 @@ void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hi
 
 // ------------------------------------------------------------------ blend
 
-// blend.slang:22-105.  One 16x16 tile per 256-thread block; Gaussians are staged in LDS 256 at a
-// time (position, conic, opacity, colour, and a skip threshold), the block stops when every
-// pixel has terminated.  skip = log(1/(255 o)) - 0.01: a power below it gives
-// o * exp(power) < 0.99/255, so the alpha < 1/255 test would discard it anyway and exp() is
-// not evaluated; the result is unchanged.
-
-
+// blend.slang:22-105.  One 16x16 tile per 256-thread block; wave w owns the 8x8 quadrant
+// (w & 1, w >> 1).  Gaussians are staged in LDS 256 at a time, the block stops when every pixel
+// has terminated.  Per pixel the operations and their order are the reference's; the exact
+// shortcuts around them change no pixel:
+//   - each staged splat is tested against each quadrant (splat_touches_rect, after a cheap
+//     bounding-box test): only a quadrant in which some pixel centre may see it with
+//     alpha >= 1/255 gets it in its list (compacted in sorted order);
+//   - skip = log(1/(255 o)) - 0.01: a power below it gives o * exp(power) < 0.99/255, so the
+//     alpha < 1/255 test would discard the splat anyway; exp() runs only if some lane needs it;
+//   - a wave leaves the round as soon as all its pixels have terminated.
+// COUNT adds up the pixel-splat pairs whose power is evaluated (for the FLOP roofline).
+template <bool COUNT>
 __global__ __launch_bounds__(kTilePixels) void k_blend(const BlendArgs a) {
+    constexpr uint32_t kWaves = kTilePixels / kWave;
+    constexpr uint32_t kQuad = 8;
     __shared__ float4 s_a[kTilePixels];  // x, y, conic a, conic b
     __shared__ float4 s_b[kTilePixels];  // conic c, opacity, skip threshold, -
     __shared__ float4 s_c[kTilePixels];  // colour
+    __shared__ uint8_t s_list[kWaves][kTilePixels];
+    __shared__ uint32_t s_cnt[kWaves][kWaves];  // [staging wave][quadrant]
 
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t tx = tile % a.gx, ty = tile / a.gx;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t pxl = tx * kTile + (tid % kTile), pyl = ty * kTile + (tid / kTile);
+    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    const uint32_t pxl = tx * kTile + (wave & 1u) * kQuad + (lane % kQuad);
+    const uint32_t pyl = ty * kTile + (wave >> 1) * kQuad + (lane / kQuad);
     const bool inside = pxl < a.width && pyl < a.height;
     const float fx = (float)pxl, fy = (float)pyl;
     const uint2 range = a.ranges[tile];
+    const uint64_t lt = lanemask_lt();
 
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !inside;
+    uint32_t evals = 0;
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(done) == (int)kTilePixels) break;
         const uint32_t k = base + tid;
+        uint32_t mask = 0;
         if (k < range.y) {
             const uint32_t g = a.vals[k];
             const float4 p = a.xyzr[g];
             const float4 c = a.conop[g];
-            const float4 col = a.rgb[g];
             s_a[tid] = make_float4(p.x, p.y, c.x, c.y);
             s_b[tid] = make_float4(c.z, c.w, __logf(1.0f / (255.0f * c.w)) - 0.01f, 0.0f);
-            s_c[tid] = col;
-        }
-        __syncthreads();
-        const uint32_t lim = min(kTilePixels, range.y - base);
-        if (!done) {
-            for (uint32_t j = 0; j < lim; ++j) {
-                const float4 A = s_a[j];
-                const float4 B = s_b[j];
-                const float dx = A.x - fx, dy = A.y - fy;
-                const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-                if (power > 0.0f) continue;
-                if (power < B.z) continue;
-                float alpha = B.y * soft_expf(power);
-                alpha = alpha < 0.99f ? alpha : 0.99f;
-                if (alpha < 1.0f / 255.0f) continue;
-                const float oma = 1.0f - alpha;
-                if (T * oma < 0.0001f) {
-                    done = true;
-                    break;
-                }
-                const float4 C = s_c[j];
-                cr = cr + (C.x * alpha) * T;
-                cg = cg + (C.y * alpha) * T;
-                cb = cb + (C.z * alpha) * T;
-                T = T * oma;
+            s_c[tid] = a.rgb[g];
+            // Bounding box of the region Q <= q0 outside which splat_touches_rect always says no
+            // (half-extents sqrt(q0 c / det), sqrt(q0 a / det), widened by 1%); only quadrants the
+            // box reaches get the exact test.  Degenerate or near-degenerate conics (|b| close to
+            // sqrt(ac), where det loses precision) give an unbounded box.
+            float ex = __builtin_inff(), ey = __builtin_inff();
+            const float det = c.x * c.z - c.y * c.y;
+            const float q0 = (__logf(255.0f * c.w) + 0.05f) * (2.0f / 0.99f);
+            if (c.x > 0.0f && c.z > 0.0f && c.y * c.y < 0.98f * (c.x * c.z) && q0 >= 0.0f) {
+                ex = 1.01f * __builtin_sqrtf(q0 * c.z / det) + 1e-3f;
+                ey = 1.01f * __builtin_sqrtf(q0 * c.x / det) + 1e-3f;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kWaves; ++q) {
+                const uint32_t x0 = tx * kTile + (q & 1u) * kQuad, y0 = ty * kTile + (q >> 1) * kQuad;
+                if (x0 >= a.width || y0 >= a.height) continue;
+                const float fx0 = (float)x0, fy0 = (float)y0;
+                const float fx1 = (float)(min(x0 + kQuad, a.width) - 1), fy1 = (float)(min(y0 + kQuad, a.height) - 1);
+                if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
+                    splat_touches_rect(p.x, p.y, c.x, c.y, c.z, c.w, fx0, fx1, fy0, fy1))
+                    mask |= 1u << q;
             }
         }
+        uint64_t bal[kWaves];
+#pragma unroll
+        for (uint32_t q = 0; q < kWaves; ++q) bal[q] = __ballot((mask >> q) & 1u);
+        if (lane < kWaves) {
+            uint64_t b = bal[0];
+#pragma unroll
+            for (uint32_t q = 1; q < kWaves; ++q) b = lane == q ? bal[q] : b;
+            s_cnt[wave][lane] = (uint32_t)__popcll(b);
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kWaves; ++q) {
+            if ((mask >> q) & 1u) {
+                uint32_t off = (uint32_t)__popcll(bal[q] & lt);
+                for (uint32_t w = 0; w < wave; ++w) off += s_cnt[w][q];
+                s_list[q][off] = (uint8_t)tid;
+            }
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; ++w) cnt += s_cnt[w][wave];
+        __syncthreads();
+
+        if (__ballot(!done) == 0) continue;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t j = s_list[wave][i];
+            const float4 A = s_a[j];
+            const float4 B = s_b[j];
+            if (COUNT) evals += done ? 0u : 1u;
+            const float dx = A.x - fx, dy = A.y - fy;
+            const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
+            const bool act = !done && !(power > 0.0f) && !(power < B.z);
+            if (__ballot(act) == 0) continue;
+            float alpha = B.y * soft_expf(power);
+            alpha = alpha < 0.99f ? alpha : 0.99f;
+            const bool ok = act && !(alpha < 1.0f / 255.0f);
+            const float tn = T * (1.0f - alpha);
+            const bool stop = ok && tn < 0.0001f;
+            const bool upd = ok && !(tn < 0.0001f);
+            const float4 C = s_c[j];
+            cr = upd ? cr + (C.x * alpha) * T : cr;
+            cg = upd ? cg + (C.y * alpha) * T : cg;
+            cb = upd ? cb + (C.z * alpha) * T : cb;
+            T = upd ? tn : T;
+            done = done || stop;
+            if (__ballot(stop) != 0 && __ballot(!done) == 0) break;
+        }
+    }
+    if (COUNT) {
+        const uint64_t s = wave_sum_u64(evals);
+        if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(a.eval_count), (unsigned long long)s);
     }
     if (inside) {
         const size_t o = (size_t)pyl * a.width + pxl;
@@ -305,7 +364,10 @@ __global__ __launch_bounds__(kTilePixels) void k_blend(const BlendArgs a) {
 
 void launch_blend(const BlendArgs& a, hipStream_t s) {
     if (a.ntiles == 0) return;
-    hipLaunchKernelGGL(k_blend, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+    if (a.eval_count)
+        hipLaunchKernelGGL(k_blend<true>, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_blend<false>, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
 }
 
 }  // namespace tpd

@@ -25,40 +25,71 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
 
 // ------------------------------------------------------------------ histograms
 
+// Digit histograms of every pass in one read of the keys.  Consecutive keys often share a digit
+// (all pairs of one Gaussian share its depth bits; neighbouring tiles share the high tile bits),
+// so each wave first collapses runs of equal digits (run heads found with one shuffle and one
+// ballot) and only run heads add their run length to the block's LDS histogram.  Each thread
+// keeps kHistUnroll loads in flight; blocks add their non-zero bins to the global counts
+// (kMaxPasses x kRadix, zero on entry; k_sort_offsets re-zeroes them after use).
+constexpr uint32_t kHistUnroll = 8;
+
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
-                                                   uint32_t n_imm, uint32_t passes, uint32_t* __restrict__ partials,
-                                                   uint32_t* __restrict__ desc0, uint32_t* __restrict__ tickets) {
+                                                   uint32_t n_imm, uint32_t passes, uint32_t tile_keys,
+                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
+                                                   uint32_t* __restrict__ tickets) {
     __shared__ uint32_t h[kMaxPasses][kRadix];
     const uint32_t n = count_ptr ? *count_ptr : n_imm;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = __lane_id();
 #pragma unroll
     for (uint32_t p = 0; p < kMaxPasses; ++p) h[p][tid] = 0;
     // The first onesweep pass needs region 0 of the look-back descriptors and all tickets at zero.
-    const uint32_t nparts = sort_parts(n);
+    const uint32_t nparts = (n + tile_keys - 1) / tile_keys;
     for (uint32_t x = blockIdx.x * 256 + tid; x < nparts * kRadix; x += gridDim.x * 256) desc0[x] = 0;
     if (blockIdx.x == 0 && tid < kMaxPasses) tickets[tid] = 0;
     __syncthreads();
 
-    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
-    const uint32_t lo = blockIdx.x * per;
-    const uint32_t hi = min(n, lo + per);
-    for (uint32_t i = lo + tid; i < hi; i += 256) {
-        const uint64_t k = keys[i];
+    constexpr uint32_t kChunk = 256 * kHistUnroll;
+    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+    for (uint32_t c0 = blockIdx.x * kChunk; c0 < n; c0 += gridDim.x * kChunk) {
+        uint64_t k[kHistUnroll];
 #pragma unroll
-        for (uint32_t p = 0; p < kMaxPasses; ++p)
-            if (p < passes) atomicAdd(&h[p][digit_of(k, p * kRadixBits)], 1u);
+        for (uint32_t u = 0; u < kHistUnroll; ++u) {
+            const uint32_t i = c0 + u * 256 + tid;
+            k[u] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kHistUnroll; ++u) {
+            const uint32_t i = c0 + u * 256 + tid;
+            const bool valid = i < n;
+            const uint64_t vmask = __ballot(valid);
+#pragma unroll
+            for (uint32_t p = 0; p < kMaxPasses; ++p) {
+                if (p >= passes) break;
+                const uint32_t d = digit_of(k[u], p * kRadixBits);
+                const uint32_t prev = __shfl_up(d, 1, 64);
+                const bool head = valid && (lane == 0 || prev != d);
+                const uint64_t heads = __ballot(head) | ~vmask;  // invalid lanes end runs
+                if (head) {
+                    const uint64_t nxt = heads & above;
+                    const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : 64u;
+                    atomicAdd(&h[p][d], end - lane);
+                }
+            }
+        }
     }
     __syncthreads();
-    for (uint32_t p = 0; p < passes; ++p) partials[((size_t)blockIdx.x * kMaxPasses + p) * kRadix + tid] = h[p][tid];
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t v = h[p][tid];
+        if (v) atomicAdd(&counts[p * kRadix + tid], v);
+    }
 }
 
-// One block per pass: sum the block histograms, exclusive scan over digits.
-__global__ __launch_bounds__(256) void k_sort_offsets(const uint32_t* __restrict__ partials, uint32_t nblocks,
-                                                      uint32_t* __restrict__ offsets) {
+// One block per pass: exclusive scan of the digit counts, which are then re-zeroed.
+__global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets) {
     __shared__ uint32_t s_w[4];
     const uint32_t p = blockIdx.x, d = threadIdx.x;
-    uint32_t s = 0;
-    for (uint32_t b = 0; b < nblocks; ++b) s += partials[((size_t)b * kMaxPasses + p) * kRadix + d];
+    const uint32_t s = counts[p * kRadix + d];
+    counts[p * kRadix + d] = 0;
     const uint32_t incl = wave_incl_scan(s);
     const uint32_t w = d / kWave;
     if (__lane_id() == kWave - 1) s_w[w] = incl;
@@ -84,45 +115,53 @@ struct OnesweepArgs {
     uint32_t* ticket;
 };
 
-__global__ __launch_bounds__(kSortBlock) void k_onesweep(const OnesweepArgs a) {
-    constexpr uint32_t kWaves = kSortBlock / kWave;
+// BLOCK threads (256 or 512) x ITEMS keys per partition.  Threads 0..255 own one digit each for
+// the per-digit bookkeeping and the look-back.
+template <uint32_t BLOCK, uint32_t ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
+    constexpr uint32_t kWaves = BLOCK / kWave;
+    constexpr uint32_t kTileKeys = BLOCK * ITEMS;
+    static_assert(BLOCK == 256 || BLOCK == 512, "one or two threads per digit");
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_whist[kWaves][kRadix];
     __shared__ uint32_t s_bstart[kRadix];
     __shared__ uint32_t s_adj[kRadix];
-    __shared__ uint32_t s_wtot[kWaves];
-    __shared__ uint64_t s_keys[kSortTile];
-    __shared__ uint8_t s_dig[kSortTile];
+    __shared__ uint32_t s_wtot[4];
+    __shared__ uint64_t s_keys[kTileKeys];
+    __shared__ uint8_t s_dig[kTileKeys];
 
     const uint32_t n = a.count_ptr ? *a.count_ptr : a.n_imm;
-    const uint32_t nparts = sort_parts(n);
+    const uint32_t nparts = (n + kTileKeys - 1) / kTileKeys;
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) s_part = atomicAdd(a.ticket, 1u);
-#pragma unroll
-    for (uint32_t w = 0; w < kWaves; ++w) s_whist[w][tid] = 0;
+    for (uint32_t x = tid; x < kWaves * kRadix; x += BLOCK) (&s_whist[0][0])[x] = 0;
     __syncthreads();
     const uint32_t part = s_part;
     if (part >= nparts) return;
-    if (a.desc_next) a.desc_next[part * kRadix + tid] = 0;
+    if (a.desc_next && tid < kRadix) a.desc_next[part * kRadix + tid] = 0;
 
-    // Load 16 keys per thread, wave-striped: element e = wave*1024 + j*64 + lane.
-    const uint32_t base = part * kSortTile;
-    uint64_t k[kSortItems];
-    uint32_t v[kSortItems];
+    // Load ITEMS keys per thread, wave-striped: element e = wave*(ITEMS*64) + j*64 + lane.
+    const uint32_t base = part * kTileKeys;
+    const uint32_t wbase = base + wave * (ITEMS * kWave) + lane;
+    uint64_t k[ITEMS];
+    uint32_t v[ITEMS];
 #pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) {
-        const uint32_t idx = base + wave * (kSortItems * kWave) + j * kWave + lane;
-        const bool ok = idx < n;
-        k[j] = ok ? a.keys_in[idx] : ~0ull;
-        v[j] = ok ? a.vals_in[idx] : 0u;
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = wbase + j * kWave;
+        k[j] = idx < n ? a.keys_in[idx] : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = wbase + j * kWave;
+        v[j] = idx < n ? a.vals_in[idx] : 0u;
     }
 
     // Rank within the wave: peers with the same digit found with 8 ballots; the lowest lane of
     // each group advances the wave's running digit counter.
     const uint64_t lt = lanemask_lt();
-    uint32_t rank[kSortItems];
+    uint32_t rank[ITEMS];
 #pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) {
+    for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = digit_of(k[j], a.shift);
         uint64_t peers = ~0ull;
 #pragma unroll
@@ -139,24 +178,29 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const OnesweepArgs a) {
     __syncthreads();
 
     // Per digit (thread = digit): wave exclusive offsets and the partition's digit count.
-    const uint32_t c0 = s_whist[0][tid], c1 = s_whist[1][tid], c2 = s_whist[2][tid], c3 = s_whist[3][tid];
-    const uint32_t tot = c0 + c1 + c2 + c3;
-    store_relaxed(&a.desc[part * kRadix + tid], (part == 0 ? kFlagP : kFlagA) | tot);
-    s_whist[0][tid] = 0;
-    s_whist[1][tid] = c0;
-    s_whist[2][tid] = c0 + c1;
-    s_whist[3][tid] = c0 + c1 + c2;
-    const uint32_t incl = wave_incl_scan(tot);
-    if (lane == kWave - 1) s_wtot[wave] = incl;
+    uint32_t tot = 0, bstart = 0;
+    if (tid < kRadix) {
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_whist[w][tid];
+            s_whist[w][tid] = tot;
+            tot += c;
+        }
+        store_relaxed(&a.desc[part * kRadix + tid], (part == 0 ? kFlagP : kFlagA) | tot);
+        const uint32_t incl = wave_incl_scan(tot);
+        if (lane == kWave - 1) s_wtot[wave] = incl;
+        bstart = incl - tot;
+    }
     __syncthreads();
-    uint32_t bstart = incl - tot;
-    for (uint32_t w = 0; w < wave; ++w) bstart += s_wtot[w];
-    s_bstart[tid] = bstart;
+    if (tid < kRadix) {
+        for (uint32_t w = 0; w < wave; ++w) bstart += s_wtot[w];
+        s_bstart[tid] = bstart;
+    }
     __syncthreads();
 
     // Reorder the partition by digit in LDS (local position = digit start + wave offset + rank).
 #pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) {
+    for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = digit_of(k[j], a.shift);
         const uint32_t lp = s_bstart[d] + s_whist[wave][d] + rank[j];
         rank[j] = lp;
@@ -164,55 +208,91 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const OnesweepArgs a) {
         s_dig[lp] = (uint8_t)d;
     }
 
-    // Decoupled look-back for digit tid.
-    uint32_t excl = 0;
-    if (part > 0) {
-        int32_t jj = (int32_t)part - 1;
-        while (true) {
-            const uint32_t dsc = load_relaxed(&a.desc[(uint32_t)jj * kRadix + tid]);
-            const uint32_t f = dsc & ~kValMask;
-            if (f == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+    // Decoupled look-back for digit tid: kLookback predecessor descriptors are loaded at once
+    // (before partition 0 a virtual inclusive prefix of 0), consumed in order up to the first
+    // inclusive prefix; an unpublished one is re-read after a short sleep.
+    if (tid < kRadix) {
+        constexpr int32_t kLookback = 8;
+        uint32_t excl = 0;
+        if (part > 0) {
+            int32_t jj = (int32_t)part - 1;
+            while (true) {
+                uint32_t dsc[kLookback];
+#pragma unroll
+                for (int32_t u = 0; u < kLookback; ++u)
+                    dsc[u] = jj - u >= 0 ? load_relaxed(&a.desc[(uint32_t)(jj - u) * kRadix + tid]) : kFlagP;
+                uint32_t add = 0;
+                int32_t used = 0;
+                bool found = false, blocked = false;
+#pragma unroll
+                for (int32_t u = 0; u < kLookback; ++u) {
+                    if (!found && !blocked) {
+                        const uint32_t f = dsc[u] & ~kValMask;
+                        if (f == 0) {
+                            blocked = true;
+                        } else {
+                            add += dsc[u] & kValMask;
+                            used = u + 1;
+                            found = f == kFlagP;
+                        }
+                    }
+                }
+                excl += add;
+                jj -= used;
+                if (found) break;
+                if (blocked) __builtin_amdgcn_s_sleep(1);
             }
-            excl += dsc & kValMask;
-            if (f == kFlagP) break;
-            --jj;
+            store_relaxed(&a.desc[part * kRadix + tid], kFlagP | (excl + tot));
         }
-        store_relaxed(&a.desc[part * kRadix + tid], kFlagP | (excl + tot));
+        s_adj[tid] = a.offsets[tid] + excl - bstart;
     }
-    s_adj[tid] = a.offsets[tid] + excl - bstart;
     __syncthreads();
 
-    const uint32_t nvalid = min(kSortTile, n - base);
+    const uint32_t nvalid = min(kTileKeys, n - base);
 #pragma unroll 4
-    for (uint32_t t = 0; t < kSortItems; ++t) {
-        const uint32_t i = t * kSortBlock + tid;
+    for (uint32_t t = 0; t < ITEMS; ++t) {
+        const uint32_t i = t * BLOCK + tid;
         if (i < nvalid) a.keys_out[s_adj[s_dig[i]] + i] = s_keys[i];
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
-    for (uint32_t j = 0; j < kSortItems; ++j) s_vals[rank[j]] = v[j];
+    for (uint32_t j = 0; j < ITEMS; ++j) s_vals[rank[j]] = v[j];
     __syncthreads();
 #pragma unroll 4
-    for (uint32_t t = 0; t < kSortItems; ++t) {
-        const uint32_t i = t * kSortBlock + tid;
+    for (uint32_t t = 0; t < ITEMS; ++t) {
+        const uint32_t i = t * BLOCK + tid;
         if (i < nvalid) a.vals_out[s_adj[s_dig[i]] + i] = s_vals[i];
     }
 }
 
+// Block shapes the tuner chooses from: {threads, keys per thread}.
+struct SortShape {
+    uint32_t block, items;
+    void (*kernel)(const OnesweepArgs);
+};
+static const SortShape kShapes[] = {
+    {256, 16, k_onesweep<256, 16>}, {256, 8, k_onesweep<256, 8>},   {256, 12, k_onesweep<256, 12>},
+    {512, 8, k_onesweep<512, 8>},   {512, 12, k_onesweep<512, 12>}, {256, 20, k_onesweep<256, 20>},
+};
+constexpr uint32_t kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+uint32_t sort_num_shapes() { return kNumShapes; }
+uint32_t sort_shape_keys(uint32_t shape) { return kShapes[shape % kNumShapes].block * kShapes[shape % kNumShapes].items; }
+
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
                        hipEvent_t after_hist) {
+    const SortShape& sh = kShapes[ws.shape % kNumShapes];
+    const uint32_t tile = sh.block * sh.items;
     const uint32_t passes = sort_passes(key_bits);
-    const uint32_t parts = sort_parts(n_max);
+    const uint32_t parts = (n_max + tile - 1) / tile;
     if (parts == 0 || passes == 0) return false;
-    uint32_t hb = (n_max + 8191u) / 8192u;
+    uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
     hb = hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
-    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, ws.partials, ws.desc,
-                       ws.tickets);
-    hipLaunchKernelGGL(k_sort_offsets, dim3(passes), dim3(256), 0, s, ws.partials, hb, ws.offsets);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, tile, ws.partials,
+                       ws.desc, ws.tickets);
+    hipLaunchKernelGGL(k_sort_offsets, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
     if (after_hist) hipEventRecord(after_hist, s);
     const uint64_t* kin = keys;
     const uint32_t* vin = vals;
@@ -227,7 +307,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * kRadix;
         o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * kRadix;
         o.ticket = ws.tickets + p;
-        hipLaunchKernelGGL(k_onesweep, dim3(parts), dim3(kSortBlock), 0, s, o);
+        hipLaunchKernelGGL(sh.kernel, dim3(parts), dim3(sh.block), 0, s, o);
         const uint64_t* tk = kout; kout = const_cast<uint64_t*>(kin); kin = tk;
         const uint32_t* tv = vout; vout = const_cast<uint32_t*>(vin); vin = tv;
     }

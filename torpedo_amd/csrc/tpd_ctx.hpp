@@ -61,6 +61,7 @@ struct tpd_ctx {
     // stage timing with HIP events on the render stream (tpd_set_profiling)
     bool profile = false;
     hipEvent_t ev[8] = {};
+    uint64_t* d_eval_count = nullptr;  // blend evaluations of the last profiled frame
 
     // backward and batched state (backward.cpp, batch.cpp)
     struct BackwardState* bwd = nullptr;

@@ -241,6 +241,47 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
     return true;
 }
 
+// Whether a splat can pass the blend's alpha >= 1/255 test at any pixel centre of the rectangle
+// [x0, x1] x [y0, y1].  Returns false only when every pixel of the rectangle is certain to skip it,
+// so dropping such splats from a tile's list leaves every pixel bit-identical.
+//
+// power = -Q/2 with Q(d) = a dx^2 + 2 b dx dy + c dy^2, d = centre - pixel; the minimum of the
+// convex Q over the rectangle is 0 if the centre lies inside, else the smallest of the four edge
+// minima (each a clamped 1-D quadratic).  Skip when 0.99 * Qmin/2 > ln(255 o) + 0.05: with
+// |b| < 0.99 sqrt(ac) the float error of Q and of the per-pixel power stays below 2e-4
+// relative, so every pixel's power is then below -ln(255 o) - 0.04 and o * exp(power) < 1/255
+// (exp is accurate to 2.5 ulp).  Anything else (NaN, non-positive-definite conic, opacity that
+// may reach 1/255) is kept.
+__device__ __forceinline__ bool splat_touches_rect(float gx, float gy, float a, float b, float c, float o, float x0,
+                                                   float x1, float y0, float y1) {
+    if (o < 1.0f / 255.0f) return false;  // alpha <= o for power <= 0
+    if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c))) return true;
+    if (gx >= x0 && gx <= x1 && gy >= y0 && gy <= y1) return true;
+    float q = __builtin_inff();
+    {
+        // vertical edges X = x0, x1: dy in [gy - y1, gy - y0]
+        const float lo = gy - y1, hi = gy - y0;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float dx = gx - (e ? x1 : x0);
+            float dy = -(b * dx) / c;
+            dy = fminf(fmaxf(dy, lo), hi);
+            q = fminf(q, (a * dx) * dx + 2.0f * (b * dx) * dy + (c * dy) * dy);
+        }
+    }
+    {
+        const float lo = gx - x1, hi = gx - x0;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float dy = gy - (e ? y1 : y0);
+            float dx = -(b * dy) / a;
+            dx = fminf(fmaxf(dx, lo), hi);
+            q = fminf(q, (a * dx) * dx + 2.0f * (b * dx) * dy + (c * dy) * dy);
+        }
+    }
+    return !(0.99f * 0.5f * q > __logf(255.0f * o) + 0.05f);
+}
+
 // ------------------------------------------------------------------ wave64 helpers
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -43,6 +43,7 @@ struct BlendArgs {
     uint32_t width, height, gx, ntiles;
     float* image;            // H x W x 3 or null
     uint8_t* rgba8;          // H x W x 4 or null
+    uint64_t* eval_count;    // when non-null, incremented by the number of evaluated pixel-splat pairs
 };
 
 void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
@@ -55,9 +56,7 @@ void launch_blend(const BlendArgs& a, hipStream_t s);
 
 constexpr uint32_t kRadixBits = 8;
 constexpr uint32_t kRadix = 1u << kRadixBits;
-constexpr uint32_t kSortBlock = 256;
-constexpr uint32_t kSortItems = 16;
-constexpr uint32_t kSortTile = kSortBlock * kSortItems;  // keys per onesweep partition
+constexpr uint32_t kSortMinTile = 2048;                    // smallest partition of any block shape
 constexpr uint32_t kMaxPasses = 8;
 constexpr uint32_t kHistBlocks = 512;                      // blocks of the digit-histogram kernel
 constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back values
@@ -65,12 +64,16 @@ constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back v
 struct SortWorkspace {
     uint32_t* desc = nullptr;      // 2 regions x parts_cap x kRadix look-back descriptors
     uint32_t parts_cap = 0;
-    uint32_t* partials = nullptr;  // kHistBlocks x kMaxPasses x kRadix block histograms
+    uint32_t* partials = nullptr;  // kMaxPasses x kRadix global digit counts (zero between sorts)
     uint32_t* offsets = nullptr;   // kMaxPasses x kRadix global exclusive digit offsets
     uint32_t* tickets = nullptr;   // kMaxPasses partition tickets
+    uint32_t shape = 0;            // onesweep block shape (index into the tuner's table)
 };
 
-__host__ __device__ inline uint32_t sort_parts(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+// Partitions for n keys at the smallest partition size (descriptor capacity for every shape).
+__host__ __device__ inline uint32_t sort_parts(uint32_t n) { return (n + kSortMinTile - 1) / kSortMinTile; }
+uint32_t sort_num_shapes();
+uint32_t sort_shape_keys(uint32_t shape);
 __host__ __device__ inline uint32_t sort_passes(uint32_t key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
 
 // Stable LSD sort of (keys, vals) by the low key_bits bits of the keys.  n comes from *count_ptr

@@ -336,6 +336,17 @@ class GaussianEngine:
         check(lib().tpd_read_ranges(self._ctx, ptr(r)))
         return r
 
+    def autotune(self) -> dict:
+        """Times every radix-pass block shape on the last frame's pairs and keeps the fastest."""
+        ms = np.zeros(64, np.float32)
+        ns = ctypes.c_uint32()
+        best = ctypes.c_uint32()
+        check(lib().tpd_autotune(self._ctx, ptr(ms), 64, ctypes.byref(ns), ctypes.byref(best)))
+        return {"sort_ms_per_shape": [round(float(x), 4) for x in ms[: ns.value]], "best_shape": int(best.value)}
+
+    def setSortShape(self, shape: int):
+        check(lib().tpd_set_sort_shape(self._ctx, int(shape)))
+
     def setProfiling(self, enable: bool = True):
         check(lib().tpd_set_profiling(self._ctx, 1 if enable else 0))
 
@@ -348,7 +359,16 @@ class GaussianEngine:
         check(lib().tpd_get_stage_times(self._ctx, ptr(ms), ctypes.byref(passes)))
         out = {k: float(v) for k, v in zip(self.STAGES, ms)}
         out["radix_passes"] = int(passes.value)
+        ev = ctypes.c_uint64()
+        check(lib().tpd_get_blend_evaluations(self._ctx, ctypes.byref(ev)))
+        out["blend_evaluations"] = int(ev.value)
         return out
+
+    def measureCopy(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """GB/s of a plain device copy (read + write bytes counted)."""
+        ms = ctypes.c_float()
+        check(lib().tpd_measure_copy(self._ctx, int(nbytes), int(iters), ctypes.byref(ms)))
+        return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     def deviceImagePtr(self) -> int:
         p = ctypes.c_void_p()
