@@ -66,6 +66,41 @@ def time_plain(eng, g, eye, w, h, steps, warmup, tune=False):
     return dt, P, med
 
 
+def time_fwd_bwd(eng, steps, warmup):
+    """Forward + backward of 3M clustered Gaussians at 1920x1080 with dL/dpixel ~ N(0,1): the span
+    from the first forward kernel to the end of the parameter-gradient pass (HIP events)."""
+    from torpedo_amd import scenes
+    W, H = 1920, 1080
+    g = scenes.cluster_scene(3_000_000, seed=3)
+    n = g.shape[0]
+    eng.uploadRecords(g, 3)
+    view, pv, fo = camera_uniforms([0.0, 0.0, -8.0], W, H)
+    eng.setCamera(view, pv, fo, W, H)
+    eng.setProfiling(True)
+    up = np.random.default_rng(3).standard_normal((H, W, 3)).astype(np.float32)
+    d_up = eng.alloc(up.nbytes)
+    eng.upload(d_up, up)
+    bufs = [eng.alloc(n * k * 4) for k in (3, 3, 4, 1, 48)]
+    for _ in range(warmup):
+        eng.render()
+        eng.backward(d_up, *bufs)
+    eng.waitIdle()
+    rec = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.render()
+        eng.backward(d_up, *bufs)
+        rec.append({**eng.stageTimes(), **eng.backwardTimes()})
+    eng.waitIdle()
+    wall = (time.perf_counter() - t0) / steps * 1000.0
+    P = eng.pairCount
+    for b in bufs + [d_up]:
+        eng.free(b)
+    med = {k: round(float(np.median([r[k] for r in rec])), 4) for k in rec[0]}
+    return {"ms": med["fwd_bwd_span"], "wall_ms_per_step": round(wall, 4), "pairs": P, "gaussians": n,
+            "width": W, "height": H, "stage_ms": {k: v for k, v in med.items() if k not in ("radix_passes",)}}
+
+
 def cpu_baseline(g, eye, w, h):
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -140,6 +175,9 @@ def main():
                 "avg_launch_us": round(t_launch_s * 1e6, 2)}
     t_launch_c2 = st["sort_passes"] / 1000.0 / max(st["radix_passes"], 1)
     sort_frac_c2 = 24.0 * P / t_launch_c2 / 1e9 / HBM_PEAK_GBS if P else 0.0
+    log("[bench] forward + backward, 3M Gaussians 1920x1080")
+    fb = time_fwd_bwd(eng, max(5, args.steps // 2), 2)
+    log(f"[bench] fwd+bwd: {fb}")
     copy_gbs = eng.measureCopy(1 << 30, 10)
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
     # Blend: ~22 FP32 operations + 1 exp per evaluated pixel-splat pair (device-counted).
@@ -172,6 +210,7 @@ def main():
                           "clusters_1M": round(sort_frac_c2, 4), "pairs_clusters_1M": P},
         "roofline": roofline,
         "roofline_blend": roofline_blend,
+        "fwd_bwd_3M_1080p": fb,
         "copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_cpu_baseline:

@@ -98,6 +98,33 @@ int tpd_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capaci
 int tpd_read_ranges(tpd_ctx* ctx, uint32_t* ranges);       /* 2 * tiles uint32: [start, end) per tile */
 /* Device pointer of the FP32 image (valid until the next render or resize). */
 int tpd_device_image(tpd_ctx* ctx, float** d_rgb);
+/* Device-to-device copy of the FP32 image into d_dst (H*W*3 floats), ordered on stream. */
+int tpd_copy_image(tpd_ctx* ctx, float* d_dst, void* stream);
+
+/* ----- backward (the reference's differentiable rasterizer is an empty stub:
+ * differential/include/torpedo/differential/GaussianRasterizer.h:1-8, README.md:8) -----
+ * Gradients of the last rendered frame with respect to the uploaded/bound records, expressed in
+ * the training parameterisation: means (N x 3), log-scales (N x 3, scale = exp(log-scale) as in
+ * GaussianGeometry.cpp:115), quaternions as fed to the kernel (N x 4, unnormalised xyzw),
+ * opacity logits (N, opacity = sigmoid(logit) as in GaussianGeometry.cpp:112) and SH (N x 48).
+ * d_dL_dimage is the upstream gradient of the FP32 image (H x W x 3, device).  All gradient
+ * pointers are device memory; with accumulate != 0 the results are added to their contents.
+ * The per-pixel transmittance is recomputed, not stored by the forward. */
+int tpd_backward(tpd_ctx* ctx, const float* d_dL_dimage, float* d_grad_means, float* d_grad_log_scales,
+                 float* d_grad_quats, float* d_grad_opacity_logits, float* d_grad_sh, int accumulate, void* stream);
+/* Per-splat screen-space gradients of the last backward (N x 9 floats: image x, y, conic a, b, c,
+ * colour r, g, b, opacity), for tests. */
+int tpd_read_splat_grads(tpd_ctx* ctx, float* g2d);
+/* Profiled backward: ms[0] splat-gradient pass, ms[1] parameter-gradient pass, ms[2] span from the
+ * start of the last profiled tpd_render to the end of this backward. */
+int tpd_get_backward_times(tpd_ctx* ctx, float* ms);
+
+/* ----- device memory for callers without another GPU runtime (TransferWorker's role,
+ * foundation/src/TransferWorker.cpp:111-158): kind 0 host->device, 1 device->host,
+ * 2 device->device; tpd_memcpy waits for the context's stream first. */
+int tpd_alloc(tpd_ctx* ctx, uint64_t bytes, void** d_ptr);
+int tpd_free(tpd_ctx* ctx, void* d_ptr);
+int tpd_memcpy(tpd_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind);
 
 /* ----- stage timing (the reference has none; SurfaceRenderer only measures whole frames) -----
  * With profiling on, tpd_render records HIP events on its stream between stages; ms[0..5]

@@ -64,6 +64,9 @@ def lib():
         L.tpo_ranges.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_uint32, _u32p]
         L.tpo_blend_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32,
                                     ctypes.c_uint32, _vp, _vp, ctypes.c_int]
+        L.tpo_blend_decisions_f32.restype = ctypes.c_uint64
+        L.tpo_blend_decisions_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32,
+                                              ctypes.c_uint32, _u32p, _vp, ctypes.c_uint64]
         L.tpo_render_f32.restype = ctypes.c_uint64
         L.tpo_render_f32.argtypes = [_f32p, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _f32p, _f32p, _f32p,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _f64p, ctypes.c_int]
@@ -235,6 +238,20 @@ def blend(splats, vals, rng, width, height, nthreads=None):
                         v if v.size else np.zeros(1, np.uint32), np.ascontiguousarray(rng, np.uint32).ravel(),
                         width, height, _ptr(img), _ptr(rgba), nthreads or nthreads_default())
     return img, rgba
+
+
+def blend_decisions(splats, vals, rng, width, height):
+    """Per pixel (row-major) the sorted positions k of the splats the FP32 blend used, in order.
+    Returns (counts[H*W], positions[total])."""
+    n = splats["color"].shape[0]
+    args = [_f32(splats["color"]).ravel(), _f32(splats["texel"]).ravel(), _f32(splats["copac"]).ravel(), n,
+            np.ascontiguousarray(vals, np.uint32) if len(vals) else np.zeros(1, np.uint32),
+            np.ascontiguousarray(rng, np.uint32).ravel(), width, height]
+    counts = np.zeros(width * height, np.uint32)
+    total = lib().tpo_blend_decisions_f32(*args, counts, None, 0)
+    pos = np.zeros(max(int(total), 1), np.uint32)
+    lib().tpo_blend_decisions_f32(*args, counts, _ptr(pos), pos.size)
+    return counts, pos[: int(total)]
 
 
 def render(gauss, cam: Camera, sh_degree=3, ent_idx=None, models=None, nthreads=None, stages=False):

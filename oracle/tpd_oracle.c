@@ -434,6 +434,47 @@ void tpo_blend_f32(const float* color, const float* texel, const float* copac, u
     free(sp);
 }
 
+/*
+ * The FP32 blend's discrete decisions (blend.slang:54-101): for every pixel (row-major), the
+ * sorted-list positions k of the splats it actually blended, in blend order.  counts[p] receives
+ * the number per pixel; positions (capacity entries, may be NULL for a counting call) the
+ * concatenated lists.  Returns the total.  Used to evaluate a dense FP64 reference on exactly the
+ * (pixel, splat) pairs the FP32 frame blended.
+ */
+uint64_t tpo_blend_decisions_f32(const float* color, const float* texel, const float* copac, uint32_t n,
+                                 const uint32_t* vals, const uint32_t* ranges, uint32_t W, uint32_t H,
+                                 uint32_t* counts, uint32_t* positions, uint64_t capacity) {
+    (void)color;
+    (void)n;
+    const uint32_t gx = (W + 15u) / 16u;
+    uint64_t total = 0;
+    for (uint32_t py = 0; py < H; ++py) {
+        for (uint32_t px = 0; px < W; ++px) {
+            const uint32_t t = (py / 16u) * gx + (px / 16u);
+            float Tr = 1.0f;
+            uint32_t cnt = 0;
+            for (uint32_t k = ranges[2 * t]; k < ranges[2 * t + 1]; ++k) {
+                const uint32_t g = vals[k];
+                const float* tx = texel + 4 * (size_t)g;
+                const float* cp = copac + 4 * (size_t)g;
+                const float dx = tx[0] - (float)px, dy = tx[1] - (float)py;
+                const float power = -0.5f * ((cp[0] * dx) * dx + (cp[2] * dy) * dy) - (cp[1] * dx) * dy;
+                if (power > 0.0f) continue;
+                float alpha = cp[3] * tpo_expf(power);
+                alpha = alpha < 0.99f ? alpha : 0.99f;
+                if (alpha < 1.0f / 255.0f) continue;
+                if (Tr * (1.0f - alpha) < 0.0001f) break;
+                Tr = Tr * (1.0f - alpha);
+                if (positions && total < capacity) positions[total] = k;
+                ++total;
+                ++cnt;
+            }
+            counts[(size_t)py * W + px] = cnt;
+        }
+    }
+    return total;
+}
+
 /* ---------------------------------------------------------------- whole pipelines */
 
 static double now_s(void) {

@@ -47,6 +47,13 @@ SIGNATURES = {
     "tpd_read_pairs": (_i32, [_vp, _vp, _vp, _u32, _i32]),
     "tpd_read_ranges": (_i32, [_vp, _vp]),
     "tpd_device_image": (_i32, [_vp, ctypes.POINTER(_vp)]),
+    "tpd_copy_image": (_i32, [_vp, _vp, _vp]),
+    "tpd_backward": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "tpd_read_splat_grads": (_i32, [_vp, _vp]),
+    "tpd_get_backward_times": (_i32, [_vp, _vp]),
+    "tpd_alloc": (_i32, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "tpd_free": (_i32, [_vp, _vp]),
+    "tpd_memcpy": (_i32, [_vp, _vp, _vp, ctypes.c_uint64, _i32]),
     "tpd_set_profiling": (_i32, [_vp, _i32]),
     "tpd_get_stage_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     "tpd_get_blend_evaluations": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64)]),

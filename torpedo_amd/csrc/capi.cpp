@@ -447,6 +447,38 @@ int tpd_get_blend_evaluations(tpd_ctx* c, uint64_t* evaluations) {
     return copy_back(c, evaluations, c->d_eval_count, sizeof(uint64_t));
 }
 
+int tpd_alloc(tpd_ctx* c, uint64_t bytes, void** d_ptr) {
+    if (!check_ctx(c) || !d_ptr) return fail(TPD_ERR_INVALID, "tpd_alloc: bad argument");
+    *d_ptr = nullptr;
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_alloc");
+}
+
+int tpd_free(tpd_ctx* c, void* d_ptr) {
+    if (!check_ctx(c)) return TPD_ERR_INVALID;
+    if (!d_ptr) return TPD_OK;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipFree(d_ptr);
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_free");
+}
+
+int tpd_memcpy(tpd_ctx* c, void* dst, const void* src, uint64_t bytes, int kind) {
+    if (!check_ctx(c) || (bytes && (!dst || !src)) || kind < 0 || kind > 2)
+        return fail(TPD_ERR_INVALID, "tpd_memcpy: bad argument");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, k);
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_memcpy");
+}
+
+int tpd_copy_image(tpd_ctx* c, float* d_dst, void* stream) {
+    if (!check_ctx(c) || !d_dst) return fail(TPD_ERR_INVALID, "tpd_copy_image: bad argument");
+    if (!c->rendered) return fail(TPD_ERR_STATE, "tpd_copy_image: nothing rendered");
+    hipError_t e = hipMemcpyAsync(d_dst, c->d_image, sizeof(float) * 3 * (size_t)c->cam.width * c->cam.height,
+                                  hipMemcpyDeviceToDevice, pick_stream(c, stream));
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_copy_image");
+}
+
 int tpd_device_image(tpd_ctx* c, float** d_rgb) {
     if (!check_ctx(c) || !d_rgb) return fail(TPD_ERR_INVALID, "tpd_device_image: bad argument");
     *d_rgb = c->d_image;

@@ -134,8 +134,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     return TPD_OK;
 }
 
-// Placeholders until the backward and batched paths allocate state of their own.
-void backward_release(tpd_ctx*) {}
+// Placeholder until the batched path allocates state of its own.
 void batch_release(tpd_ctx*) {}
 
 }  // namespace tpd

@@ -8,6 +8,11 @@
 #include <string>
 #include <vector>
 
+namespace tpd {
+struct BackwardState;
+struct BatchState;
+}  // namespace tpd
+
 struct tpd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -64,8 +69,8 @@ struct tpd_ctx {
     uint64_t* d_eval_count = nullptr;  // blend evaluations of the last profiled frame
 
     // backward and batched state (backward.cpp, batch.cpp)
-    struct BackwardState* bwd = nullptr;
-    struct BatchState* batch = nullptr;
+    tpd::BackwardState* bwd = nullptr;
+    tpd::BatchState* batch = nullptr;
 };
 
 namespace tpd {

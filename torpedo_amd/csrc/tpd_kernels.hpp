@@ -46,6 +46,40 @@ struct BlendArgs {
     uint64_t* eval_count;    // when non-null, incremented by the number of evaluated pixel-splat pairs
 };
 
+struct BackwardArgs {
+    const uint2* ranges;
+    const uint32_t* vals;
+    const float4* xyzr;
+    const float4* conop;
+    const float4* rgb;
+    uint32_t width, height, gx, gy, ntiles;
+    const float* dL_dimg;    // H x W x 3
+    const uint32_t* offsets; // per-Gaussian exclusive scan of tile counts (pre-sort pair position)
+    float* partials;         // P x 9 per (tile, splat) pair at its pre-sort position: dL/d(image x, y,
+                             // conic a, b, c, r, g, b, opacity); zero on entry
+};
+
+struct PreBackwardArgs {
+    const float* gauss;
+    const uint32_t* ent_idx;
+    const EntityMats* ent;
+    CameraParams cam;
+    uint32_t n, sh_degree;
+    const uint32_t* tiles;
+    const uint32_t* offsets;
+    const float* partials;       // P x 9, summed per Gaussian over its tiles in tile order
+    float* g2d;                  // N x 9 per-Gaussian sums (written, for inspection)
+    float* grad_means;           // N x 3
+    float* grad_log_scales;      // N x 3
+    float* grad_quats;           // N x 4
+    float* grad_opacity_logits;  // N
+    float* grad_sh;              // N x 48
+    uint32_t accumulate;         // add to the outputs instead of overwriting them
+};
+
+void launch_blend_bwd(const BackwardArgs& a, hipStream_t s);
+void launch_preprocess_bwd(const PreBackwardArgs& a, hipStream_t s);
+
 void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
 void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
 void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,

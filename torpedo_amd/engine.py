@@ -272,6 +272,12 @@ class GaussianEngine:
         self._n = n
         self._transform_host = TransformHost(self, {Entity(0): 0})
 
+    def bindDeviceScene(self, d_records: int, n: int, sh_degree: int = 3):
+        """Renders from n 60-float records already in device memory (borrowed pointer)."""
+        check(lib().tpd_bind_device_scene(self._ctx, ctypes.c_void_p(d_records), int(n), min(int(sh_degree), 3)))
+        self._n = int(n)
+        self._transform_host = TransformHost(self, {Entity(0): 0})
+
     def getTransformHost(self) -> TransformHost:
         return self._transform_host
 
@@ -287,7 +293,44 @@ class GaussianEngine:
         self.render(stream)
 
     def render(self, stream=None):
-        check(lib().tpd_render(self._ctx, stream))
+        check(lib().tpd_render(self._ctx, None if stream is None else ctypes.c_void_p(stream)))
+        self.frameCount = getattr(self, "frameCount", 0) + 1
+
+    def copyImageTo(self, d_dst: int, stream=None):
+        check(lib().tpd_copy_image(self._ctx, ctypes.c_void_p(d_dst), None if stream is None else ctypes.c_void_p(stream)))
+
+    def backward(self, d_dL_dimage: int, d_means: int, d_log_scales: int, d_quats: int, d_opacity_logits: int,
+                 d_sh: int, accumulate: bool = False, stream=None):
+        """tpd_backward on device pointers (gradients of the last rendered frame)."""
+        vp = ctypes.c_void_p
+        check(lib().tpd_backward(self._ctx, vp(d_dL_dimage), vp(d_means), vp(d_log_scales), vp(d_quats),
+                                 vp(d_opacity_logits), vp(d_sh), 1 if accumulate else 0,
+                                 None if stream is None else vp(stream)))
+
+    def backwardTimes(self) -> dict:
+        ms = np.zeros(3, np.float32)
+        check(lib().tpd_get_backward_times(self._ctx, ptr(ms)))
+        return {"splat_grad_pass": float(ms[0]), "param_grad_pass": float(ms[1]), "fwd_bwd_span": float(ms[2])}
+
+    def alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        check(lib().tpd_alloc(self._ctx, int(nbytes), ctypes.byref(p)))
+        return int(p.value)
+
+    def free(self, d: int):
+        check(lib().tpd_free(self._ctx, ctypes.c_void_p(d)))
+
+    def upload(self, d: int, host: np.ndarray):
+        h = np.ascontiguousarray(host)
+        check(lib().tpd_memcpy(self._ctx, ctypes.c_void_p(d), ptr(h), h.nbytes, 0))
+
+    def download(self, host: np.ndarray, d: int):
+        check(lib().tpd_memcpy(self._ctx, ptr(host), ctypes.c_void_p(d), host.nbytes, 1))
+
+    def readSplatGrads(self) -> np.ndarray:
+        out = np.empty((max(self._n, 1), 9), np.float32)
+        check(lib().tpd_read_splat_grads(self._ctx, ptr(out)))
+        return out[: self._n]
 
     def waitIdle(self):
         """Engine::waitIdle (Engine.h:19)."""
