@@ -151,6 +151,79 @@ int tpd_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_sh
 /* Selects a radix-pass block shape by index (0 = default). */
 int tpd_set_sort_shape(tpd_ctx* ctx, uint32_t shape);
 
+/* ----- batched rendering -------------------------------------------------------------------------
+ * The reference renders one camera of one compiled scene per GaussianEngine::rasterFrame
+ * (GaussianEngine.h:26; one Scene per compile, GaussianEngine.cpp:359-399) and has no batched form.
+ * A batch holds several scenes in the context and renders many views of them (any resolutions) with
+ * one launch per stage, recorded once into a HIP graph and replayed.  Every view's pairs, ranges and
+ * image are those of a plain tpd_render of that scene and camera. */
+
+/* One view of a batch: the scene it shows and the camera uniforms of tpd_set_camera. */
+typedef struct tpd_view {
+    uint32_t scene;
+    uint32_t width, height;
+    float view[16];
+    float proj_view[16];
+    float focal_ndc[2];
+} tpd_view;
+
+/* One scene's parameter-gradient buffers (device; layouts as tpd_backward). */
+typedef struct tpd_scene_grads {
+    float* means;           /* N x 3 */
+    float* log_scales;      /* N x 3 */
+    float* quats;           /* N x 4 */
+    float* opacity_logits;  /* N */
+    float* sh;              /* N x 48 */
+} tpd_scene_grads;
+
+/* Scene::add + GaussianEngine::compile for one more scene of the batch (Scene.h:26-62,
+ * GaussianEngine.cpp:359-399, transforms :442-486); arguments as tpd_upload.  *scene_id receives
+ * the scene's index (0, 1, ... in call order). */
+int tpd_scene_add(tpd_ctx* ctx, const void* gaussians, uint32_t n, const uint32_t* entity_idx, const float* models,
+                  uint32_t n_ent, uint32_t sh_degree, uint32_t* scene_id);
+/* As tpd_scene_add with records already in device memory (borrowed, 16-byte aligned, N x 60 floats;
+ * they are read by every batch frame and must stay valid). */
+int tpd_scene_add_device(tpd_ctx* ctx, const float* d_gaussians, uint32_t n, uint32_t sh_degree, uint32_t* scene_id);
+/* TransformHost::transform (TransformHost.cpp:3-12) for an entity of a batch scene. */
+int tpd_scene_set_transform(tpd_ctx* ctx, uint32_t scene, uint32_t entity, const float model[16]);
+/* Removes every batch scene and view and frees the batch's buffers. */
+int tpd_scene_clear(tpd_ctx* ctx);
+/* Sets the batch's views (image v of the batch is views[v]).  Changing only the cameras keeps the
+ * recorded frame; changing a view's scene or size re-lays the batch out.  Waits for a running frame. */
+int tpd_batch_set_views(tpd_ctx* ctx, const tpd_view* views, uint32_t nviews);
+/* enabled = 0 issues the batch's kernels one by one instead of replaying the recorded graph. */
+int tpd_batch_set_graph(tpd_ctx* ctx, int enabled);
+/* Launches a batch frame (asynchronous).  for_backward != 0 keeps the per-view state that
+ * tpd_batch_backward needs.  tpd_batch_synchronize or any batch read waits for the frame and, if its
+ * pairs overflowed the buffers, grows them and renders it again. */
+int tpd_batch_render(tpd_ctx* ctx, int for_backward, void* stream);
+int tpd_batch_synchronize(tpd_ctx* ctx);
+/* Gradients of the last batch frame (rendered with for_backward) for dL/dimages d_dL_dimages
+ * (all views' H x W x 3 images concatenated in view order, as tpd_batch_device_images) into the
+ * per-scene buffers grads[0 .. number of scenes); accumulate != 0 adds to them. */
+int tpd_batch_backward(tpd_ctx* ctx, const float* d_dL_dimages, const tpd_scene_grads* grads, int accumulate,
+                       void* stream);
+/* Forward + backward of the batch as one recorded graph (asynchronous, validated as tpd_batch_render). */
+int tpd_batch_step(tpd_ctx* ctx, const float* d_dL_dimages, const tpd_scene_grads* grads, int accumulate, void* stream);
+/* Pair count of the last batch frame (all views). */
+int tpd_batch_get_pair_count(tpd_ctx* ctx, uint32_t* P);
+/* Where view `view` lies in the batch's index spaces: its first global tile (keys carry global
+ * tiles), its first item (pair values are items; item - item_base = Gaussian index) and its first
+ * pixel in the concatenated images. */
+int tpd_batch_view_layout(tpd_ctx* ctx, uint32_t view, uint32_t* tile_base, uint32_t* item_base, uint64_t* pix_base);
+/* Readbacks of one view's image (H x W x 3 float, H x W x 4 UNORM8). */
+int tpd_batch_read_image_f32(tpd_ctx* ctx, uint32_t view, float* host);
+int tpd_batch_read_image_rgba8(tpd_ctx* ctx, uint32_t view, uint8_t* host);
+/* Device pointer of the concatenated float images (owned by the context) and their total pixels. */
+int tpd_batch_device_images(tpd_ctx* ctx, float** d_images, uint64_t* total_pixels);
+/* The sorted global (key, value) pairs of the last batch frame; *P receives the count. */
+int tpd_batch_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capacity, uint32_t* P);
+/* With tpd_set_profiling on: ms[0] = GPU time of the last batch frame; for frames issued without the
+ * graph also ms[1..5] = count pass, preprocess + duplicate, sort, ranges, blend (else -1). */
+int tpd_batch_get_times(tpd_ctx* ctx, float* ms);
+/* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
+int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
+
 /* ----- host camera math (torpedo/math, rendering/src/Camera.cpp, extension/src/PerspectiveCamera.cpp) ----- */
 /* Camera::lookAt (Camera.cpp:3-16) */
 void tpd_look_at(const float eye[3], const float center[3], const float up[3], float view[16]);

@@ -92,12 +92,14 @@ def test_backward_accumulate_and_empty(eng, orc):
     eng.bindDeviceScene(rec.data_ptr(), 800, 3)
     eng.setCamera(cam.view, cam.proj_view, cam.focal_ndc, 64, 48)
     eng.render()
-    up = torch.randn(48, 64, 3, device="cuda")
-    outs = [torch.zeros(800, k, device="cuda") for k in (3, 3, 4, 1, 48)]
+    up = torch.randn(48, 64, 3, device="cuda", dtype=torch.float32)
+    outs = [torch.zeros(800, k, device="cuda", dtype=torch.float32) for k in (3, 3, 4, 1, 48)]
+    torch.cuda.synchronize()  # the engine runs on its own stream
     for acc in (False, True):
         eng.backward(up.data_ptr(), *[o.data_ptr() for o in outs], accumulate=acc)
     eng.waitIdle()
-    once = [torch.zeros(800, k, device="cuda") for k in (3, 3, 4, 1, 48)]
+    once = [torch.zeros(800, k, device="cuda", dtype=torch.float32) for k in (3, 3, 4, 1, 48)]
+    torch.cuda.synchronize()
     eng.backward(up.data_ptr(), *[o.data_ptr() for o in once])
     eng.waitIdle()
     # float atomics sum in a run-dependent order: equal up to rounding, not bit for bit
@@ -110,7 +112,8 @@ def test_backward_accumulate_and_empty(eng, orc):
     rec2 = pack_records(*p2)
     eng.bindDeviceScene(rec2.data_ptr(), 100, 3)
     eng.render()
-    outs2 = [torch.ones(100, k, device="cuda") for k in (3, 3, 4, 1, 48)]
+    outs2 = [torch.ones(100, k, device="cuda", dtype=torch.float32) for k in (3, 3, 4, 1, 48)]
+    torch.cuda.synchronize()
     eng.backward(up.data_ptr(), *[o.data_ptr() for o in outs2])
     eng.waitIdle()
     assert all(float(o.abs().max()) == 0.0 for o in outs2)

@@ -1,0 +1,277 @@
+"""Batched rendering on the GPU (through the C ABI): every view of a batch must equal the plain
+frame of the same scene and camera -- per-view sorted keys (global tile - the view's first tile)
+and values (item - the view's first item), FP32 image and RGBA8 image all bit-exact -- and the
+plain frame is itself pinned bit-exact to the CPU oracle by test_gpu_forward.py; one view per
+case is also compared with the oracle directly.  Batched gradients equal the sum over the scene's
+views of the plain backward (FP32, relative error below 1e-5: only the summation order differs).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import small_scene
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def engines():
+    from torpedo_amd.engine import GaussianEngine
+    b = GaussianEngine()
+    p = GaussianEngine()
+    yield b, p
+    b.close()
+    p.close()
+
+
+def view_tuple(scene_id, eye, w, h, up=(0.0, -1.0, 0.0)):
+    from torpedo_amd.engine import PerspectiveCamera
+    cam = PerspectiveCamera(w, h)
+    cam.lookAt(list(eye), [0.0, 0.0, 0.0], list(up))
+    v, pv, fo = cam.uniforms()
+    return (scene_id, v, pv, fo, w, h)
+
+
+def plain_frame(plain, view):
+    _, v, pv, fo, w, h = view
+    plain.setCamera(v, pv, fo, w, h)
+    plain.render()
+    k, vals = plain.readPairs()
+    return k, vals, plain.readImage(), plain.readRGBA8()
+
+
+def check_batch(beng, plain, records, degs, views, name, orc=None):
+    """records[s], degs[s]: scene s as added to beng; compares every view with the plain frame."""
+    keys, vals = beng.readBatchPairs()
+    P = beng.batchPairCount
+    say(f"[batch] {name}: {len(views)} views, P={P}")
+    assert keys.shape[0] == P
+    tiles = (keys >> np.uint64(32)).astype(np.int64)
+    assert np.all(np.diff(tiles) >= 0), "global tiles not sorted"
+    covered = 0
+    for vi, view in enumerate(views):
+        sid, _, _, _, w, h = view
+        tb, ib, pb = beng.batchViewLayout(vi)
+        nt = ((w + 15) // 16) * ((h + 15) // 16)
+        lo, hi = np.searchsorted(tiles, tb), np.searchsorted(tiles, tb + nt)
+        kv, vv = keys[lo:hi], vals[lo:hi]
+        local_k = ((kv >> np.uint64(32)) - np.uint64(tb)) << np.uint64(32) | (kv & np.uint64(0xFFFFFFFF))
+        local_v = vv - np.uint32(ib)
+        plain.uploadRecords(records[sid], degs[sid])
+        pk, pv, pimg, prgba = plain_frame(plain, view)
+        assert np.array_equal(local_k, pk), (name, vi, "keys")
+        assert np.array_equal(local_v, pv), (name, vi, "values")
+        img = beng.readBatchImage(vi)
+        assert np.array_equal(img.view(np.uint32), pimg.view(np.uint32)), (name, vi, "image")
+        assert np.array_equal(beng.readBatchRGBA8(vi), prgba), (name, vi, "rgba8")
+        covered += hi - lo
+        if orc is not None and vi == 0:
+            _, v, pvm, fo, w, h = view
+            ref = orc.render(records[sid], orc.Camera(v, pvm, fo, w, h), degs[sid])
+            assert np.array_equal(local_k, ref["keys"]) and np.array_equal(local_v, ref["vals"]), (name, "oracle")
+            assert np.abs(img - ref["image"]).max() <= 1e-5, (name, "oracle image")
+    assert covered == P, (name, covered, P)
+
+
+def test_batch_views_of_one_scene(engines, orc):
+    from torpedo_amd import scenes
+    beng, plain = engines
+    beng.clearScenes()
+    g = scenes.cluster_scene(30_000, seed=7)
+    sid = beng.addScene(g)
+    sizes = [(320, 240), (257, 131), (64, 64), (512, 288), (100, 700), (1, 1), (1280, 720)]
+    eyes = scenes.fibonacci_sphere(len(sizes), 8.0)
+    views = [view_tuple(sid, eyes[i], w, h, up=(0.0, 0.0, 1.0)) for i, (w, h) in enumerate(sizes)]
+    beng.setViews(views)
+    for graph in (True, False, True):
+        beng.setBatchGraph(graph)
+        beng.rasterBatch()
+        beng.syncBatch()
+        check_batch(beng, plain, [g], [3], views, f"one-scene graph={graph}", orc=orc if graph else None)
+
+
+def test_batch_uneven_scenes(engines):
+    beng, plain = engines
+    beng.clearScenes()
+    sizes = [0, 1, 255, 257, 5000, 20000]
+    degs = [3, 0, 1, 2, 3, 3]
+    recs = [small_scene(n, seed=40 + i, spread=1.0, log_scale_mean=-3.5) for i, n in enumerate(sizes)]
+    from torpedo_amd.engine import Settings
+    ids = []
+    for r, d in zip(recs, degs):
+        s = Settings()
+        s.sphericalHarmonicsDegree = d
+        ids.append(beng.addScene(r, s))
+    # scene 3 has no view; views interleave scenes
+    plan = [(5, (200, 120)), (0, (64, 48)), (4, (333, 77)), (1, (16, 16)), (5, (96, 96)), (2, (128, 256)),
+            (4, (640, 360)), (5, (31, 33))]
+    views = [view_tuple(ids[s], (0.3 * k, -0.2, -3.0 + 0.1 * k), w, h) for k, (s, (w, h)) in enumerate(plan)]
+    beng.setViews(views)
+    beng.setBatchGraph(True)
+    beng.rasterBatch()
+    check_batch(beng, plain, recs, degs, views, "uneven scenes")
+
+
+def test_batch_camera_update_and_growth(engines):
+    beng, plain = engines
+    beng.clearScenes()
+    g = small_scene(20_000, seed=9, log_scale_mean=-3.5)  # means in [-1, 1]^3
+    sid = beng.addScene(g)
+    far = [view_tuple(sid, (0.0, 0.0, -20.0 - 5 * i), w, h) for i, (w, h) in enumerate([(256, 256), (400, 300)])]
+    near = [view_tuple(sid, (0.2 * i, 0.0, -2.5), w, h) for i, (w, h) in enumerate([(256, 256), (400, 300)])]
+    beng.setBatchGraph(True)
+    beng.setViews(far)
+    beng.rasterBatch()
+    p_far = beng.batchPairCount
+    check_batch(beng, plain, [g], [3], far, "far cameras")
+    # same sizes: only the view table changes; the recorded frame overflows and is rendered again
+    beng.setViews(near)
+    beng.rasterBatch()
+    p_near = beng.batchPairCount
+    say(f"[batch] pairs far {p_far} -> near {p_near}")
+    assert p_near > 2 * p_far
+    check_batch(beng, plain, [g], [3], near, "near cameras after growth")
+    for _ in range(3):  # replays give the same frame
+        beng.rasterBatch()
+    check_batch(beng, plain, [g], [3], near, "near cameras replayed")
+
+
+def test_batch_entities_with_transforms(engines):
+    from torpedo_amd.engine import Scene
+    beng, plain = engines
+    beng.clearScenes()
+    a = small_scene(3000, seed=61)
+    b = small_scene(2000, seed=62)
+    sc = Scene()
+    ea = sc.add(a)
+    eb = sc.add(b)
+    M = np.eye(4, dtype=np.float32)
+    M[0, 3], M[1, 3] = 0.4, -0.3
+    M[:3, :3] = np.array([[0.9, -0.2, 0.0], [0.2, 0.9, 0.1], [0.0, -0.1, 1.1]], np.float32)
+    sid = beng.addScene(sc)
+    beng.setSceneTransform(sid, eb, M)
+    views = [view_tuple(sid, (0.2, 0.1, -3.0), 160, 120), view_tuple(sid, (-1.0, 0.5, -2.5), 200, 100)]
+    beng.setViews(views)
+    beng.rasterBatch()
+    keys, vals = beng.readBatchPairs()
+    plain.compile(sc)
+    plain.getTransformHost().transform(eb, M)
+    for vi, view in enumerate(views):
+        tb, ib, _ = beng.batchViewLayout(vi)
+        w, h = view[4], view[5]
+        nt = ((w + 15) // 16) * ((h + 15) // 16)
+        t = (keys >> np.uint64(32)).astype(np.int64)
+        lo, hi = np.searchsorted(t, tb), np.searchsorted(t, tb + nt)
+        pk, pv, pimg, _ = plain_frame(plain, view)
+        lk = ((keys[lo:hi] >> np.uint64(32)) - np.uint64(tb)) << np.uint64(32) | (keys[lo:hi] & np.uint64(0xFFFFFFFF))
+        assert np.array_equal(lk, pk) and np.array_equal(vals[lo:hi] - np.uint32(ib), pv), vi
+        assert np.array_equal(beng.readBatchImage(vi), pimg), vi
+    del ea
+
+
+def test_batch_without_gaussians(engines):
+    beng, _ = engines
+    beng.clearScenes()
+    sid = beng.addScene(np.zeros((0, 60), np.float32))
+    views = [view_tuple(sid, (0, 0, -3), 48, 40), view_tuple(sid, (1, 0, -3), 17, 5)]
+    beng.setViews(views)
+    beng.rasterBatch()
+    assert beng.batchPairCount == 0
+    for vi in range(2):
+        assert not beng.readBatchImage(vi).any()
+        rgba = beng.readBatchRGBA8(vi)
+        assert not rgba[..., :3].any() and np.all(rgba[..., 3] == 255)
+
+
+def _plain_grads(plain, rec, deg, views, dls):
+    """Sum over views of the plain backward for the scene rec."""
+    n = rec.shape[0]
+    plain.uploadRecords(rec, deg)
+    bufs = [plain.alloc(max(n, 1) * k * 4) for k in (3, 3, 4, 1, 48)]
+    d_dl = plain.alloc(max(max(d.nbytes for d in dls), 4))
+    for k, (view, dl) in enumerate(zip(views, dls)):
+        _, v, pv, fo, w, h = view
+        plain.setCamera(v, pv, fo, w, h)
+        plain.render()
+        plain.upload(d_dl, dl)
+        plain.backward(d_dl, *bufs, accumulate=k > 0)
+    plain.waitIdle()
+    out = []
+    for b, k in zip(bufs, (3, 3, 4, 1, 48)):
+        h = np.empty((n, k), np.float32)
+        if n:
+            plain.download(h, b)
+        out.append(h)
+        plain.free(b)
+    plain.free(d_dl)
+    return out
+
+
+def _rel(a, b):
+    d = np.linalg.norm(a.astype(np.float64) - b.astype(np.float64))
+    s = np.linalg.norm(b.astype(np.float64))
+    return d / s if s > 0 else d
+
+
+@pytest.mark.parametrize("use_step", [False, True])
+def test_batch_backward_equals_sum_of_plain(engines, use_step):
+    beng, plain = engines
+    beng.clearScenes()
+    recs = [small_scene(2500, seed=71, log_scale_mean=-3.0), small_scene(1200, seed=72, log_scale_mean=-3.0)]
+    degs = [3, 2]
+    from torpedo_amd.engine import Settings
+    ids = []
+    for r, d in zip(recs, degs):
+        s = Settings()
+        s.sphericalHarmonicsDegree = d
+        ids.append(beng.addScene(r, s))
+    plan = [(0, (96, 64), (0.1, -0.2, -3.0)), (1, (80, 80), (0.0, 0.3, -2.8)), (0, (64, 100), (-0.6, 0.2, -2.9)),
+            (0, (50, 40), (0.5, 0.5, -3.2))]
+    views = [view_tuple(ids[s], eye, w, h) for s, (w, h), eye in plan]
+    beng.setViews(views)
+    rng = np.random.default_rng(5)
+    dls = [rng.normal(size=(h, w, 3)).astype(np.float32) for _, (w, h), _ in plan]
+    dl_all = np.concatenate([d.reshape(-1) for d in dls])
+    d_dl = beng.alloc(dl_all.nbytes)
+    beng.upload(d_dl, dl_all)
+    grads = []
+    host = []
+    for r in recs:
+        n = r.shape[0]
+        bufs = [beng.alloc(n * k * 4) for k in (3, 3, 4, 1, 48)]
+        grads.append(bufs)
+    beng.setBatchGraph(True)
+    for rep in range(2):
+        if use_step:
+            beng.batchStep(d_dl, grads, accumulate=rep > 0)
+            beng.syncBatch()
+        else:
+            beng.rasterBatch(for_backward=True)
+            beng.batchBackward(d_dl, grads, accumulate=rep > 0)
+        beng.waitIdle()
+    for s, r in enumerate(recs):
+        n = r.shape[0]
+        got = []
+        for b, k in zip(grads[s], (3, 3, 4, 1, 48)):
+            h = np.empty((n, k), np.float32)
+            beng.download(h, b)
+            got.append(h)
+        sv = [i for i, (sc, _, _) in enumerate(plan) if sc == s]
+        ref = _plain_grads(plain, r, degs[s], [views[i] for i in sv], [dls[i] for i in sv])
+        for name, gg, rr in zip(("means", "log_scales", "quats", "opacity_logits", "sh"), got, ref):
+            e = _rel(gg, 2.0 * rr)  # two accumulated steps
+            say(f"[batch-bwd] step={use_step} scene {s} {name}: rel err {e:.2e}")
+            assert e < GRAD_TOL, (s, name, e)
+        host.append(got)
+    for bufs in grads:
+        for b in bufs:
+            beng.free(b)
+    beng.free(d_dl)

@@ -13,7 +13,15 @@ import torch
 
 from conftest import small_scene
 
-torch.set_default_dtype(torch.float64)
+
+
+@pytest.fixture(autouse=True)
+def float64_default():
+    """FP64 default for tensors made inside these tests only (restored afterwards)."""
+    prev = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(prev)
 
 GROUPS = ("means", "log_scales", "quats", "opacity_logits", "sh")
 

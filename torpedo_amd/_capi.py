@@ -29,6 +29,16 @@ class TpdConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("keep_unsorted", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
 
 
+class TpdView(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("view", ctypes.c_float * 16), ("proj_view", ctypes.c_float * 16), ("focal_ndc", ctypes.c_float * 2)]
+
+
+class TpdSceneGrads(ctypes.Structure):
+    _fields_ = [("means", ctypes.c_void_p), ("log_scales", ctypes.c_void_p), ("quats", ctypes.c_void_p),
+                ("opacity_logits", ctypes.c_void_p), ("sh", ctypes.c_void_p)]
+
+
 SIGNATURES = {
     "tpd_last_error": (ctypes.c_char_p, []),
     "tpd_create": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(TpdConfig)]),
@@ -61,6 +71,25 @@ SIGNATURES = {
     "tpd_sort_pairs": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(_i32)]),
     "tpd_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_set_sort_shape": (_i32, [_vp, _u32]),
+    "tpd_scene_add": (_i32, [_vp, _vp, _u32, _vp, _vp, _u32, _u32, ctypes.POINTER(_u32)]),
+    "tpd_scene_add_device": (_i32, [_vp, _vp, _u32, _u32, ctypes.POINTER(_u32)]),
+    "tpd_scene_set_transform": (_i32, [_vp, _u32, _u32, _vp]),
+    "tpd_scene_clear": (_i32, [_vp]),
+    "tpd_batch_set_views": (_i32, [_vp, ctypes.POINTER(TpdView), _u32]),
+    "tpd_batch_set_graph": (_i32, [_vp, _i32]),
+    "tpd_batch_render": (_i32, [_vp, _i32, _vp]),
+    "tpd_batch_synchronize": (_i32, [_vp]),
+    "tpd_batch_backward": (_i32, [_vp, _vp, ctypes.POINTER(TpdSceneGrads), _i32, _vp]),
+    "tpd_batch_step": (_i32, [_vp, _vp, ctypes.POINTER(TpdSceneGrads), _i32, _vp]),
+    "tpd_batch_get_pair_count": (_i32, [_vp, ctypes.POINTER(_u32)]),
+    "tpd_batch_view_layout": (_i32, [_vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
+                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "tpd_batch_read_image_f32": (_i32, [_vp, _u32, _vp]),
+    "tpd_batch_read_image_rgba8": (_i32, [_vp, _u32, _vp]),
+    "tpd_batch_device_images": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
+    "tpd_batch_read_pairs": (_i32, [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32)]),
+    "tpd_batch_get_times": (_i32, [_vp, _vp]),
+    "tpd_batch_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_look_at": (None, [_vp, _vp, _vp, _vp]),
     "tpd_perspective": (None, [_u32, _u32, _f32, _f32, _f32, _vp]),
     "tpd_fy_from_fov": (_f32, [_f32]),

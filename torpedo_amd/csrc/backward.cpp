@@ -62,8 +62,6 @@ extern "C" int tpd_backward(tpd_ctx* c, const float* d_dL_dimage, float* d_grad_
             if (!ev && (e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
         hipEventRecord(b->ev[0], s);
     }
-    if ((e = hipMemsetAsync(b->partials, 0, 9 * (size_t)c->last_pairs * sizeof(float), s)) != hipSuccess)
-        return hip_fail(e, "clear backward buffers");
     const uint32_t W = c->cam.width, H = c->cam.height;
     const uint32_t gx = (W + kTile - 1) / kTile, gy = (H + kTile - 1) / kTile;
     if (c->last_pairs > 0) {

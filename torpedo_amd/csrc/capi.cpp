@@ -25,33 +25,6 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(TPD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Grows a device allocation to hold at least `count` elements (contents are not preserved).
-template <typename T>
-static hipError_t grow(T*& p, size_t& cap, size_t count, double headroom = 1.0) {
-    if (p && cap >= count) return hipSuccess;
-    if (p) {
-        hipError_t e = hipFree(p);
-        if (e != hipSuccess) return e;
-        p = nullptr;
-    }
-    size_t want = (size_t)((double)count * headroom);
-    want = std::max<size_t>(want, std::max<size_t>(count, 1));
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
-    if (e != hipSuccess) {
-        cap = 0;
-        p = nullptr;
-        return e;
-    }
-    cap = want;
-    return hipSuccess;
-}
-
-template <typename T>
-static void release(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
 uint32_t bit_length(uint32_t v) {
     uint32_t b = 0;
     while (v) {

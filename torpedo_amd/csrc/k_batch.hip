@@ -1,0 +1,285 @@
+// Batched rendering: many views of many scenes, one launch per stage.  The reference renders one
+// camera per GaussianEngine::rasterFrame (GaussianEngine.h:26) and has no batched form; each view
+// of a batch is rendered exactly as the plain frame renders it (same per-Gaussian projection,
+// same per-view key order, same per-tile blend), only the index spaces are concatenated (the
+// layout is described with BatchArgs).
+//
+//   k_batch_mats            entity matrices of every (view, entity)
+//   k_batch_count           per (preprocess block, view) pair counts; decoupled look-back over
+//                           the blocks turns them into pair positions; total P and the overflow
+//                           flag; clears the tile ranges
+//   k_batch_preprocess      each block reads its 256 Gaussian records once and, for every view of
+//                           its scene, projects them, writes the splats and emits the pairs
+//   (sort, ranges)          the plain path's kernels over the global keys
+//   k_batch_blend           one 16x16 tile of some view per block (blend_tile)
+//   k_batch_blend_bwd       blend_bwd_tile per tile
+//   k_batch_preprocess_bwd  per Gaussian, the chain rule summed over the views of its scene, into
+//                           the scene's gradient buffers
+#include "tpd_blend.hpp"
+#include "tpd_grad.hpp"
+#include "tpd_kernels.hpp"
+
+namespace tpd {
+
+__global__ void k_batch_mats(const BatchSceneDev* __restrict__ scenes, const BatchViewDev* __restrict__ views,
+                             EntityMats* __restrict__ mats) {
+    const BatchViewDev& vw = views[blockIdx.y];
+    const BatchSceneDev& sc = scenes[vw.scene];
+    const uint32_t ne = sc.n_ent ? sc.n_ent : 1u;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    entity_mats(sc.models ? sc.models + 16 * (size_t)e : nullptr, vw.cam, mats[vw.mat_base + e]);
+}
+
+void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, uint32_t nviews, uint32_t max_ent,
+                       EntityMats* mats, hipStream_t s) {
+    if (nviews == 0) return;
+    const uint32_t ne = max_ent ? max_ent : 1u;
+    hipLaunchKernelGGL(k_batch_mats, dim3((ne + 63) / 64, nviews), dim3(64), 0, s, scenes, views, mats);
+}
+
+__device__ __forceinline__ bool project_in_view(const float* rec, const EntityMats& m, const BatchViewDev& vw,
+                                                uint32_t sh_degree, Projected& p) {
+    return project_gaussian(rec, m.vm, m.pvm, vw.cam.view, (0.5f * (float)vw.cam.width) * vw.cam.focal_ndc[0],
+                            (0.5f * (float)vw.cam.height) * vw.cam.focal_ndc[1], vw.cam.width, vw.cam.height, vw.gx,
+                            vw.gy, sh_degree, p);
+}
+
+// ------------------------------------------------------------------ count pass
+
+__global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a) {
+    constexpr uint32_t kWaves = kPreBlock / kWave;
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_w[kWaves][kWave];  // wave sums of the current chunk of 64 views
+
+    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
+    for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
+        a.ranges[t] = make_uint2(0u, 0u);
+    if (a.nblocks == 0 && blockIdx.x == 0 && tid == 0) {  // no Gaussians: the one block reports P = 0
+        a.counters[kCtrPairs] = 0;
+        a.counters[kCtrOverflow] = 0;
+    }
+    __syncthreads();
+    const uint32_t b = s_part;
+    if (b >= a.nblocks) return;  // tickets are reset every frame; true only without Gaussians
+    const BatchSceneDev& sc = a.scenes[a.block_scene[b]];
+    const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
+    const uint32_t g = lb * kPreBlock + tid;
+    const bool valid = g < sc.n;
+    const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
+    const float* rec = sc.gauss + (size_t)(valid ? g : 0u) * 60;
+    const uint32_t entry0 = sc.entry_base + lb * nviews;
+
+    // Per view: the block's pair count, stored in bases[] until the look-back gives positions.
+    uint32_t my_total = 0;  // threads 0..63: sum of the counts they stored
+    for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
+        const uint32_t kn = min(kWave, nviews - k0);
+        for (uint32_t kk = 0; kk < kn; ++kk) {
+            const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k0 + kk]];
+            uint32_t t = 0;
+            if (valid) {
+                Projected p;
+                project_in_view(rec, a.mats[vw.mat_base + e], vw, sc.sh_degree, p);
+                t = p.tiles;
+            }
+            const uint32_t ws = wave_incl_scan(t);
+            if (lane == kWave - 1) s_w[wave][kk] = ws;
+        }
+        __syncthreads();
+        if (tid < kn) {
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kWaves; ++w) c += s_w[w][tid];
+            a.bases[entry0 + k0 + tid] = c;
+            my_total += c;
+        }
+        __syncthreads();
+    }
+
+    if (wave == 0) {
+        const uint64_t total = wave_sum_u64(my_total);
+        if (lane == 0) store_relaxed(&a.scan_desc[b], (b == 0 ? kDescP : kDescA) | total);
+        uint64_t excl = 0;
+        if (b > 0) excl = scan_lookback(a.scan_desc, b);
+        if (lane == 0) {
+            if (b > 0) store_relaxed(&a.scan_desc[b], kDescP | (excl + total));
+            if (b == a.nblocks - 1) {
+                const uint64_t P = excl + total;
+                a.counters[kCtrPairs] = P > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)P;
+                a.counters[kCtrOverflow] = P > a.cap ? 1u : 0u;
+            }
+        }
+        // counts -> first pair position of each (block, view), views in the scene's order
+        uint64_t carry = excl;
+        for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
+            const uint32_t k = k0 + lane;
+            const uint32_t c = k < nviews ? a.bases[entry0 + k] : 0u;  // stored by this same thread
+            const uint32_t incl = wave_incl_scan(c);
+            if (k < nviews) {
+                const uint64_t pos = carry + incl - c;
+                a.bases[entry0 + k] = pos > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)pos;
+            }
+            carry += __shfl(incl, kWave - 1, kWave);
+        }
+    }
+}
+
+void launch_batch_count(const BatchArgs& a, hipStream_t s) {
+    const uint32_t nb = a.nblocks ? a.nblocks : 1u;  // one block clears the ranges of an empty batch
+    hipLaunchKernelGGL(k_batch_count, dim3(nb), dim3(kPreBlock), 0, s, a);
+}
+
+// ------------------------------------------------------------------ preprocess + duplicate
+
+__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs a) {
+    __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
+    __shared__ uint32_t s_incl[kPreBlock];
+    __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
+    __shared__ uint32_t s_depth[kPreBlock];
+
+    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    const uint32_t b = blockIdx.x;
+    const BatchSceneDev& sc = a.scenes[a.block_scene[b]];
+    const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
+    const uint32_t g = lb * kPreBlock + tid;
+    const bool valid = g < sc.n;
+    const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
+    const uint32_t entry0 = sc.entry_base + lb * nviews;
+    const uint32_t sh_degree = sc.sh_degree;
+
+    // The record is read once and kept in registers for every view.
+    float rec[60];
+    if (valid) {
+        const float4* r4 = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
+#pragma unroll
+        for (int q = 0; q < 15; ++q) {
+            const float4 t = r4[q];
+            rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 60; ++q) rec[q] = 0.0f;
+    }
+
+    for (uint32_t k = 0; k < nviews; ++k) {
+        const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k]];
+        const uint32_t item = vw.item_base + g;
+        Projected p;
+        p.tiles = 0;
+        if (valid) {
+            if (project_in_view(rec, a.mats[vw.mat_base + e], vw, sh_degree, p)) {
+                a.xyzr[item] = make_float4(p.px, p.py, p.vz, p.radius);
+                a.conop[item] = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
+                a.rgb[item] = make_float4(p.r, p.g, p.b, 0.0f);
+            }
+            if (a.tiles) a.tiles[item] = p.tiles;
+        }
+        // block-wide inclusive scan of tile counts
+        const uint32_t incl_w = wave_incl_scan(p.tiles);
+        if (lane == kWave - 1) s_wave_sum[wave] = incl_w;
+        __syncthreads();
+        uint32_t wave_off = 0, block_total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+            const uint32_t v = s_wave_sum[w];
+            wave_off += (w < wave) ? v : 0u;
+            block_total += v;
+        }
+        const uint32_t incl = wave_off + incl_w;
+        s_incl[tid] = incl;
+        if (p.tiles) {
+            s_x0[tid] = p.x0;
+            s_y0[tid] = p.y0;
+            s_w[tid] = p.x1 - p.x0;
+            s_depth[tid] = as_uint(p.vz);
+        }
+        const uint32_t base = a.bases[entry0 + k];
+        if (a.offsets && valid) a.offsets[item] = base + incl - p.tiles;
+        __syncthreads();
+        emit_block_pairs(s_incl, s_x0, s_y0, s_w, s_depth, block_total, base, vw.gx, vw.tile_base, a.cap, a.keys,
+                         a.vals, vw.item_base + lb * kPreBlock);
+        __syncthreads();
+    }
+}
+
+void launch_batch_preprocess(const BatchArgs& a, hipStream_t s) {
+    if (a.nblocks == 0) return;
+    hipLaunchKernelGGL(k_batch_preprocess, dim3(a.nblocks), dim3(kPreBlock), 0, s, a);
+}
+
+// ------------------------------------------------------------------ blend, forward and backward
+
+__device__ __forceinline__ TileView batch_tile_view(const BatchBlendArgs& a, uint32_t tile) {
+    const BatchViewDev& vw = a.views[a.tile_view[tile]];
+    const uint32_t local = tile - vw.tile_base;
+    return TileView{local % vw.gx, local / vw.gx, vw.cam.width, vw.cam.height, vw.gx, vw.gy, vw.pix_base};
+}
+
+__global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArgs a) {
+    __shared__ BlendLds lds;
+    if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const TileView tv = batch_tile_view(a, tile);
+    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.image, a.rgba8, nullptr);
+}
+
+void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s) {
+    if (a.ntiles == 0) return;
+    hipLaunchKernelGGL(k_batch_blend, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+}
+
+__global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlendArgs a) {
+    __shared__ BlendBwdLds lds;
+    if (a.counters[kCtrOverflow]) return;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const TileView tv = batch_tile_view(a, tile);
+    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.dL_dimg, a.offsets, a.partials);
+}
+
+void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s) {
+    if (a.ntiles == 0) return;
+    hipLaunchKernelGGL(k_batch_blend_bwd, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+}
+
+// ------------------------------------------------------------------ preprocess backward
+
+__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchPreBwdArgs a) {
+    if (a.counters[kCtrOverflow]) return;  // gradients are left untouched; the step is run again
+    const uint32_t b = blockIdx.x;
+    const uint32_t si = a.block_scene[b];
+    const BatchSceneDev& sc = a.scenes[si];
+    const uint32_t g = (b - sc.first_block) * kPreBlock + threadIdx.x;
+    if (g >= sc.n) return;
+    const float* rec = sc.gauss + (size_t)g * 60;
+    const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
+    float gm[3] = {0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f}, gop = 0.f;
+    float gsh[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k) gsh[k] = 0.f;
+    for (uint32_t k = 0; k < sc.nviews; ++k) {
+        const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k]];
+        const uint32_t item = vw.item_base + g;
+        const uint32_t tiles = a.tiles[item];
+        if (tiles == 0) continue;
+        float gr[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
+        const float* src = a.partials + (size_t)a.offsets[item] * 9;
+        for (uint32_t t = 0; t < tiles; ++t)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) gr[q] += src[t * 9 + q];
+        const EntityMats& m = a.mats[vw.mat_base + e];
+        accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, gm, gls, gq, gop, gsh);
+    }
+    const BatchGradsDev& gd = a.grads[si];
+    store_param_grads(g, a.accumulate != 0, gm, gls, gq, gop, gsh, gd.means, gd.log_scales, gd.quats,
+                      gd.opacity_logits, gd.sh);
+}
+
+void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s) {
+    if (a.nblocks == 0) return;
+    hipLaunchKernelGGL(k_batch_preprocess_bwd, dim3(a.nblocks), dim3(kPreBlock), 0, s, a);
+}
+
+}  // namespace tpd

@@ -23,6 +23,13 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
     return (uint32_t)(k >> shift) & (kRadix - 1u);
 }
 
+// Keys are (high word << 32) | low word.  When the low word is known to use only lo_bits bits
+// (a positive float depth has bit 31 clear), the sort runs on the compacted key
+// (high << lo_bits) | low, which orders exactly as the full key and can need one pass less.
+__device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits) {
+    return ((k >> 32) << lo_bits) | (k & ((1ull << lo_bits) - 1ull));
+}
+
 // ------------------------------------------------------------------ histograms
 
 // Digit histograms of every pass in one read of the keys.  Consecutive keys often share a digit
@@ -34,11 +41,11 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
 constexpr uint32_t kHistUnroll = 8;
 
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
-                                                   uint32_t n_imm, uint32_t passes, uint32_t tile_keys,
+                                                   uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t tile_keys,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
                                                    uint32_t* __restrict__ tickets) {
     __shared__ uint32_t h[kMaxPasses][kRadix];
-    const uint32_t n = count_ptr ? *count_ptr : n_imm;
+    const uint32_t n = count_ptr ? min(*count_ptr, n_imm) : n_imm;  // device count, capped by the capacity
     const uint32_t tid = threadIdx.x, lane = __lane_id();
 #pragma unroll
     for (uint32_t p = 0; p < kMaxPasses; ++p) h[p][tid] = 0;
@@ -57,6 +64,8 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
             const uint32_t i = c0 + u * 256 + tid;
             k[u] = i < n ? keys[i] : 0ull;
         }
+#pragma unroll
+        for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits);
 #pragma unroll
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
@@ -109,6 +118,7 @@ struct OnesweepArgs {
     const uint32_t* count_ptr;
     uint32_t n_imm;
     uint32_t shift;
+    uint32_t lo_bits;          // bits of the key's low word (32, or 31 for positive depths)
     const uint32_t* offsets;   // kRadix global exclusive offsets of this pass's digits
     uint32_t* desc;            // this pass's look-back region (parts x kRadix), zero on entry
     uint32_t* desc_next;       // the other region, cleared here for the next pass
@@ -130,7 +140,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     __shared__ uint64_t s_keys[kTileKeys];
     __shared__ uint8_t s_dig[kTileKeys];
 
-    const uint32_t n = a.count_ptr ? *a.count_ptr : a.n_imm;
+    const uint32_t n = a.count_ptr ? min(*a.count_ptr, a.n_imm) : a.n_imm;
     const uint32_t nparts = (n + kTileKeys - 1) / kTileKeys;
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) s_part = atomicAdd(a.ticket, 1u);
@@ -159,10 +169,12 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // Rank within the wave: peers with the same digit found with 8 ballots; the lowest lane of
     // each group advances the wave's running digit counter.
     const uint64_t lt = lanemask_lt();
-    uint32_t rank[ITEMS];
+    uint32_t rank[ITEMS], dig[ITEMS];
+#pragma unroll
+    for (uint32_t j = 0; j < ITEMS; ++j) dig[j] = digit_of(compact_key(k[j], a.lo_bits), a.shift);
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t d = digit_of(k[j], a.shift);
+        const uint32_t d = dig[j];
         uint64_t peers = ~0ull;
 #pragma unroll
         for (uint32_t b = 0; b < kRadixBits; ++b) {
@@ -201,7 +213,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // Reorder the partition by digit in LDS (local position = digit start + wave offset + rank).
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t d = digit_of(k[j], a.shift);
+        const uint32_t d = dig[j];
         const uint32_t lp = s_bstart[d] + s_whist[wave][d] + rank[j];
         rank[j] = lp;
         s_keys[lp] = k[j];
@@ -282,7 +294,7 @@ uint32_t sort_shape_keys(uint32_t shape) { return kShapes[shape % kNumShapes].bl
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist) {
+                       hipEvent_t after_hist, uint32_t lo_bits) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t tile = sh.block * sh.items;
     const uint32_t passes = sort_passes(key_bits);
@@ -290,7 +302,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     if (parts == 0 || passes == 0) return false;
     uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
     hb = hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
-    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, tile, ws.partials,
+    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, tile, ws.partials,
                        ws.desc, ws.tickets);
     hipLaunchKernelGGL(k_sort_offsets, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
     if (after_hist) hipEventRecord(after_hist, s);
@@ -303,6 +315,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
         o.count_ptr = count_ptr; o.n_imm = n_max;
         o.shift = p * kRadixBits;
+        o.lo_bits = lo_bits;
         o.offsets = ws.offsets + p * kRadix;
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * kRadix;
         o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * kRadix;

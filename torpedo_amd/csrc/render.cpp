@@ -134,7 +134,4 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     return TPD_OK;
 }
 
-// Placeholder until the batched path allocates state of its own.
-void batch_release(tpd_ctx*) {}
-
 }  // namespace tpd

@@ -85,4 +85,32 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs);
 int render_forward(tpd_ctx* c, hipStream_t s);
 void backward_release(tpd_ctx* c);
 void batch_release(tpd_ctx* c);
+
+// Grows a device allocation to hold at least `count` elements (contents are not preserved).
+template <typename T>
+inline hipError_t grow(T*& p, size_t& cap, size_t count, double headroom = 1.0) {
+    if (p && cap >= count) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return e;
+        p = nullptr;
+    }
+    size_t want = (size_t)((double)count * headroom);
+    if (want < count) want = count;
+    if (want < 1) want = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+    if (e != hipSuccess) {
+        cap = 0;
+        p = nullptr;
+        return e;
+    }
+    cap = want;
+    return hipSuccess;
+}
+
+template <typename T>
+inline void release(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
 }  // namespace tpd

@@ -351,4 +351,95 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
     return xcd < r ? xcd * (q + 1u) + pos : r * (q + 1u) + (xcd - r) * q + pos;
 }
 
+// ------------------------------------------------------------------ shared kernel pieces
+
+// VM = view * model, PVM = projView * model (Slang mul order: each entry summed k = 0..3 left to
+// right; project.slang:48/57 via common.slang:102,106).  models null = identity.
+__device__ __forceinline__ void entity_mats(const float* __restrict__ model, const CameraParams& cam, EntityMats& out) {
+    float M[16];
+    if (model) {
+        for (int i = 0; i < 16; ++i) M[i] = model[i];
+    } else {
+        for (int i = 0; i < 16; ++i) M[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            out.vm[i * 4 + j] = ((cam.view[i * 4 + 0] * M[0 * 4 + j] + cam.view[i * 4 + 1] * M[1 * 4 + j])
+                                 + cam.view[i * 4 + 2] * M[2 * 4 + j]) + cam.view[i * 4 + 3] * M[3 * 4 + j];
+            out.pvm[i * 4 + j] = ((cam.proj_view[i * 4 + 0] * M[0 * 4 + j] + cam.proj_view[i * 4 + 1] * M[1 * 4 + j])
+                                  + cam.proj_view[i * 4 + 2] * M[2 * 4 + j]) + cam.proj_view[i * 4 + 3] * M[3 * 4 + j];
+        }
+}
+
+// Scan look-back descriptor: bits 63..62 = state (0 none, 1 aggregate, 2 inclusive prefix),
+// bits 61..0 = value.  One descriptor per partition.
+constexpr uint64_t kDescA = 1ull << 62;
+constexpr uint64_t kDescP = 2ull << 62;
+constexpr uint64_t kDescMask = (1ull << 62) - 1;
+
+// Exclusive prefix of partition `part` from its predecessors' descriptors; called by one whole
+// wave (lane l inspects partition part-1-l, 64 at a time).  Partitions are numbered in the order
+// their blocks took an atomic ticket, so every predecessor is running and publishes eventually.
+__device__ __forceinline__ uint64_t scan_lookback(uint64_t* desc, uint32_t part) {
+    uint64_t excl = 0;
+    int64_t j = (int64_t)part - 1;
+    const uint32_t l = __lane_id();
+    while (true) {
+        const int64_t idx = j - (int64_t)l;
+        const uint64_t d = idx >= 0 ? load_relaxed(&desc[idx]) : kDescP;
+        const uint32_t st = (uint32_t)(d >> 62);
+        const uint64_t mP = __ballot(st == 2);
+        const uint64_t mX = __ballot(st == 0);
+        const uint32_t first_p = mP ? (uint32_t)__builtin_ctzll(mP) : 64u;
+        const uint64_t upto = first_p >= 63 ? ~0ull : ((2ull << first_p) - 1ull);
+        if (mX & upto) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint64_t mine = (l <= first_p) ? (d & kDescMask) : 0ull;
+        excl += wave_sum_u64(mine);
+        if (first_p < 64) return excl;
+        j -= 64;
+    }
+}
+
+// Duplicate (keygen.slang:21-54) done by the whole block: pair j of the block belongs to the
+// thread whose inclusive tile-count offset s_incl first exceeds j, and covers that thread's
+// rectangle row-major.  Writes key ((tile_base + tile) << 32) | depth bits and value
+// val_first + owner at position base + j when below cap.
+__device__ __forceinline__ void emit_block_pairs(const uint32_t* s_incl, const int32_t* s_x0, const int32_t* s_y0,
+                                                 const int32_t* s_w, const uint32_t* s_depth, uint32_t block_total,
+                                                 uint64_t base, uint32_t gx, uint32_t tile_base, uint64_t cap,
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                 uint32_t val_first) {
+    for (uint32_t j = threadIdx.x; j < block_total; j += kPreBlock) {
+        uint32_t lo = 0, hi = kPreBlock - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_incl[mid] > j) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t owner = lo;
+        const uint32_t own_tiles = s_incl[owner] - (owner ? s_incl[owner - 1] : 0u);
+        const uint32_t k = j - (s_incl[owner] - own_tiles);
+        const uint32_t w = (uint32_t)s_w[owner];
+        // k / w through a float reciprocal, corrected by one step either way (exact below 2^22)
+        uint32_t ry, rx;
+        if (k < (1u << 22)) {
+            int32_t q = (int32_t)((float)k * (1.0f / (float)w));
+            int32_t r = (int32_t)k - q * (int32_t)w;
+            if (r < 0) { --q; r += (int32_t)w; }
+            if (r >= (int32_t)w) { ++q; r -= (int32_t)w; }
+            ry = (uint32_t)q; rx = (uint32_t)r;
+        } else {
+            ry = k / w; rx = k - ry * w;
+        }
+        const uint32_t tile = tile_base + ((uint32_t)s_y0[owner] + ry) * gx + ((uint32_t)s_x0[owner] + rx);
+        const uint64_t gpos = base + j;
+        if (gpos < cap) {
+            keys[gpos] = ((uint64_t)tile << 32) | s_depth[owner];
+            vals[gpos] = val_first + owner;
+        }
+    }
+}
+
 }  // namespace tpd

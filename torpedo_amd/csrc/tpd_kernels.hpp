@@ -56,7 +56,7 @@ struct BackwardArgs {
     const float* dL_dimg;    // H x W x 3
     const uint32_t* offsets; // per-Gaussian exclusive scan of tile counts (pre-sort pair position)
     float* partials;         // P x 9 per (tile, splat) pair at its pre-sort position: dL/d(image x, y,
-                             // conic a, b, c, r, g, b, opacity); zero on entry
+                             // conic a, b, c, r, g, b, opacity); every pair is written
 };
 
 struct PreBackwardArgs {
@@ -82,9 +82,115 @@ void launch_preprocess_bwd(const PreBackwardArgs& a, hipStream_t s);
 
 void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
 void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
+// Tile ranges of the sorted keys; P = min(counters[kCtrPairs], cap).  Also re-zeroes the scan
+// descriptors and the scan ticket for the next frame.
 void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
-                   uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s);
+                   uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s,
+                   uint32_t cap = 0xFFFFFFFFu);
 void launch_blend(const BlendArgs& a, hipStream_t s);
+
+// ------------------------------------------------------------------ batched rendering
+//
+// Many views of many scenes in one launch per stage.  Layout:
+//   items   view-major: Gaussian g of view v is item views[v].item_base + g; per-item splats,
+//           tile counts and pair offsets are indexed by item, pair values are items;
+//   tiles   global tile t of view v is views[v].tile_base + (local tile); keys carry global
+//           tiles, so one sort orders every view's pairs and the views' ranges are disjoint;
+//   pixels  view v's image starts at pixel views[v].pix_base of the concatenated images;
+//   blocks  preprocess block b covers Gaussians [256 (b - first_block), +256) of scene
+//           block_scene[b] and projects them into every view of that scene, reading each
+//           Gaussian record once; the (block, view) partitions get their pair positions from the
+//           count pass (table `bases`, ordered block-major, then the scene's views in order, so
+//           each view's pairs keep Gaussian order).
+
+struct BatchSceneDev {
+    const float* gauss;          // N x 60 records
+    const uint32_t* ent_idx;     // N entity indices, or null
+    const float* models;         // n_ent x 16 model matrices, or null (identity)
+    uint32_t n, sh_degree, n_ent;
+    uint32_t first_block;        // first preprocess block of the scene
+    uint32_t view_begin, nviews; // the scene's views: scene_views[view_begin, view_begin + nviews)
+    uint32_t entry_base;         // (block b, k-th view) entry = entry_base + (b - first_block) * nviews + k
+    uint32_t pad;
+};
+
+struct BatchViewDev {
+    CameraParams cam;
+    uint32_t scene, gx, gy, tile_base;
+    uint32_t item_base;          // item of Gaussian g = item_base + g
+    uint32_t mat_base;           // entity matrices of this view at mats[mat_base + e]
+    uint64_t pix_base;           // first pixel of this view in the concatenated images
+};
+
+struct BatchArgs {
+    const BatchSceneDev* scenes;
+    const BatchViewDev* views;
+    const uint32_t* scene_views;  // view indices grouped by scene
+    const uint32_t* block_scene;  // scene of each preprocess block
+    uint32_t nblocks;
+    const EntityMats* mats;
+    uint32_t* bases;              // per (block, view) entry: first pair position (count pass output)
+    uint64_t* scan_desc;          // nblocks look-back descriptors, zero on entry
+    uint32_t* counters;           // kCtr* block
+    uint32_t cap;                 // pair capacity
+    float4* xyzr;                 // per item (written for visible items only)
+    float4* conop;
+    float4* rgb;
+    uint32_t* tiles;              // per item tile counts, or null (kept for the backward)
+    uint32_t* offsets;            // per item pre-sort pair offsets, or null
+    uint64_t* keys;
+    uint32_t* vals;
+    uint2* ranges;                // all views' tiles, cleared by the count pass
+    uint32_t ntiles;
+};
+
+struct BatchBlendArgs {
+    const BatchViewDev* views;
+    const uint32_t* tile_view;    // view of each global tile
+    const uint2* ranges;
+    const uint32_t* vals;
+    const float4* xyzr;
+    const float4* conop;
+    const float4* rgb;
+    uint32_t ntiles;
+    const uint32_t* counters;     // the frame's kCtr* block: nothing is done after an overflow
+    float* image;                 // concatenated H x W x 3 images, or null
+    uint8_t* rgba8;               // concatenated H x W x 4 images, or null
+    const float* dL_dimg;         // backward: concatenated H x W x 3 upstream gradients
+    const uint32_t* offsets;      // backward: per item pre-sort pair offsets
+    float* partials;              // backward: P x 9 per pair
+};
+
+struct BatchGradsDev {            // one scene's parameter-gradient buffers
+    float* means;                 // N x 3
+    float* log_scales;            // N x 3
+    float* quats;                 // N x 4
+    float* opacity_logits;        // N
+    float* sh;                    // N x 48
+};
+
+struct BatchPreBwdArgs {
+    const BatchSceneDev* scenes;
+    const BatchViewDev* views;
+    const uint32_t* scene_views;
+    const uint32_t* block_scene;
+    uint32_t nblocks;
+    const EntityMats* mats;
+    const uint32_t* tiles;
+    const uint32_t* offsets;
+    const float* partials;
+    const BatchGradsDev* grads;   // per scene
+    const uint32_t* counters;     // nothing is written after an overflow
+    uint32_t accumulate;
+};
+
+void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, uint32_t nviews, uint32_t max_ent,
+                       EntityMats* mats, hipStream_t s);
+void launch_batch_count(const BatchArgs& a, hipStream_t s);
+void launch_batch_preprocess(const BatchArgs& a, hipStream_t s);
+void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
+void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s);
+void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s);
 
 // ------------------------------------------------------------------ radix sort
 
@@ -111,11 +217,14 @@ uint32_t sort_shape_keys(uint32_t shape);
 __host__ __device__ inline uint32_t sort_passes(uint32_t key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
 
 // Stable LSD sort of (keys, vals) by the low key_bits bits of the keys.  n comes from *count_ptr
-// when count_ptr is non-null (device-resident count, grids sized by n_max), else from n_max.
+// when count_ptr is non-null (device-resident count capped at n_max, grids sized by n_max), else
+// from n_max.
 // Returns true when the sorted data ended in the alternate buffers.
 // after_hist (optional) is recorded between the histogram kernels and the first onesweep pass.
+// lo_bits < 32 sorts the compacted key (high word << lo_bits) | low word, valid when every key's
+// low word fits in lo_bits bits; key_bits then counts the compacted key's bits.
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist = nullptr);
+                       hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32);
 
 }  // namespace tpd

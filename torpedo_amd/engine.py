@@ -418,6 +418,149 @@ class GaussianEngine:
         check(lib().tpd_device_image(self._ctx, ctypes.byref(p)))
         return int(p.value or 0)
 
+    # -- batched rendering ---------------------------------------------------------------------
+    # The reference renders one camera of one compiled scene per rasterFrame (GaussianEngine.h:26);
+    # a batch holds several scenes and renders many views of them per launch (tpd_batch_*).
+    def addScene(self, scene, settings: Settings | None = None) -> int:
+        """Adds a Scene (or an (N, 60) record array) to the batch; returns its scene id."""
+        sid = ctypes.c_uint32()
+        if isinstance(scene, Scene):
+            deg = min(int((settings or Settings.getDefault()).sphericalHarmonicsDegree), 3)
+            data = scene.dataAll()
+            emap = scene.buildEntityMap()
+            n_ent = max(len(emap), 1)
+            idx = scene.entityIndices() if len(emap) > 1 else None
+            models = np.tile(np.eye(4, dtype=np.float32).reshape(1, 16), (n_ent, 1)) if idx is not None else None
+            n = scene.countAll()
+            check(lib().tpd_scene_add(self._ctx, ptr(data) if n else None, n, ptr(idx), ptr(models),
+                                      n_ent if idx is not None else 0, deg, ctypes.byref(sid)))
+            self._batch_maps = getattr(self, "_batch_maps", {})
+            self._batch_maps[int(sid.value)] = dict(emap)
+        else:
+            deg = min(int(settings.sphericalHarmonicsDegree), 3) if settings else 3
+            data = f32(scene).reshape(-1, GAUSSIAN_FLOATS)
+            n = data.shape[0]
+            check(lib().tpd_scene_add(self._ctx, ptr(data) if n else None, n, None, None, 0, deg, ctypes.byref(sid)))
+        self._batch_sizes = getattr(self, "_batch_sizes", []) + [n]
+        return int(sid.value)
+
+    def addDeviceScene(self, d_records: int, n: int, sh_degree: int = 3) -> int:
+        """Adds n records already in device memory (borrowed, 16-byte aligned) to the batch."""
+        sid = ctypes.c_uint32()
+        check(lib().tpd_scene_add_device(self._ctx, ctypes.c_void_p(d_records), int(n), min(int(sh_degree), 3),
+                                         ctypes.byref(sid)))
+        self._batch_sizes = getattr(self, "_batch_sizes", []) + [int(n)]
+        return int(sid.value)
+
+    def setSceneTransform(self, scene_id: int, entity, matrix):
+        """TransformHost::transform for an entity of batch scene scene_id."""
+        emap = getattr(self, "_batch_maps", {}).get(int(scene_id), {})
+        if entity not in emap:
+            return  # TransformHost.cpp:4-6
+        m = f32(matrix).reshape(16)
+        check(lib().tpd_scene_set_transform(self._ctx, int(scene_id), emap[entity], ptr(m)))
+
+    def clearScenes(self):
+        check(lib().tpd_scene_clear(self._ctx))
+        self._batch_sizes, self._batch_views, self._batch_maps = [], [], {}
+
+    def setViews(self, views):
+        """views: list of (scene_id, PerspectiveCamera) or (scene_id, view, proj_view, focal_ndc, w, h)."""
+        arr = (_capi.TpdView * max(len(views), 1))()
+        sizes = []
+        for i, v in enumerate(views):
+            if len(v) == 2:
+                sid, cam = v
+                view, pv, fo = cam.uniforms()
+                w, h = cam.width, cam.height
+            else:
+                sid, view, pv, fo, w, h = v
+            arr[i].scene = int(sid)
+            arr[i].width, arr[i].height = int(w), int(h)
+            arr[i].view[:] = [float(x) for x in f32(view).reshape(16)]
+            arr[i].proj_view[:] = [float(x) for x in f32(pv).reshape(16)]
+            arr[i].focal_ndc[:] = [float(x) for x in f32(fo).reshape(2)]
+            sizes.append((int(w), int(h)))
+        check(lib().tpd_batch_set_views(self._ctx, arr, len(views)))
+        self._batch_views = sizes
+
+    def setBatchGraph(self, enabled: bool = True):
+        check(lib().tpd_batch_set_graph(self._ctx, 1 if enabled else 0))
+
+    def rasterBatch(self, for_backward: bool = False, stream=None):
+        """Launches one batch frame (asynchronous; syncBatch or any batch read waits for it)."""
+        check(lib().tpd_batch_render(self._ctx, 1 if for_backward else 0,
+                                     None if stream is None else ctypes.c_void_p(stream)))
+
+    def syncBatch(self):
+        check(lib().tpd_batch_synchronize(self._ctx))
+
+    @staticmethod
+    def _grads_table(grads):
+        t = (_capi.TpdSceneGrads * max(len(grads), 1))()
+        for i, g in enumerate(grads):
+            t[i].means, t[i].log_scales, t[i].quats, t[i].opacity_logits, t[i].sh = [int(x) for x in g]
+        return t
+
+    def batchBackward(self, d_dL_dimages: int, grads, accumulate: bool = False, stream=None):
+        """grads: per scene (d_means, d_log_scales, d_quats, d_opacity_logits, d_sh) device pointers."""
+        check(lib().tpd_batch_backward(self._ctx, ctypes.c_void_p(d_dL_dimages), self._grads_table(grads),
+                                       1 if accumulate else 0, None if stream is None else ctypes.c_void_p(stream)))
+
+    def batchStep(self, d_dL_dimages: int, grads, accumulate: bool = False, stream=None):
+        """Forward + backward of the batch in one recorded graph (asynchronous)."""
+        check(lib().tpd_batch_step(self._ctx, ctypes.c_void_p(d_dL_dimages), self._grads_table(grads),
+                                   1 if accumulate else 0, None if stream is None else ctypes.c_void_p(stream)))
+
+    @property
+    def batchPairCount(self) -> int:
+        v = ctypes.c_uint32()
+        check(lib().tpd_batch_get_pair_count(self._ctx, ctypes.byref(v)))
+        return int(v.value)
+
+    def batchViewLayout(self, view: int):
+        tb, ib, pb = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().tpd_batch_view_layout(self._ctx, int(view), ctypes.byref(tb), ctypes.byref(ib), ctypes.byref(pb)))
+        return int(tb.value), int(ib.value), int(pb.value)
+
+    def readBatchImage(self, view: int) -> np.ndarray:
+        w, h = self._batch_views[view]
+        img = np.empty((h, w, 3), np.float32)
+        check(lib().tpd_batch_read_image_f32(self._ctx, int(view), ptr(img)))
+        return img
+
+    def readBatchRGBA8(self, view: int) -> np.ndarray:
+        w, h = self._batch_views[view]
+        img = np.empty((h, w, 4), np.uint8)
+        check(lib().tpd_batch_read_image_rgba8(self._ctx, int(view), ptr(img)))
+        return img
+
+    def batchImagesPtr(self):
+        p, n = ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib().tpd_batch_device_images(self._ctx, ctypes.byref(p), ctypes.byref(n)))
+        return int(p.value or 0), int(n.value)
+
+    def readBatchPairs(self):
+        P = self.batchPairCount
+        k = np.empty(max(P, 1), np.uint64)
+        v = np.empty(max(P, 1), np.uint32)
+        n = ctypes.c_uint32()
+        check(lib().tpd_batch_read_pairs(self._ctx, ptr(k), ptr(v), max(P, 1), ctypes.byref(n)))
+        return k[:P], v[:P]
+
+    BATCH_STAGES = ("frame", "count", "preprocess_duplicate", "sort", "ranges", "blend")
+
+    def batchTimes(self) -> dict:
+        ms = np.zeros(6, np.float32)
+        check(lib().tpd_batch_get_times(self._ctx, ptr(ms)))
+        return {k: float(v) for k, v in zip(self.BATCH_STAGES, ms) if v >= 0}
+
+    def autotuneBatch(self) -> dict:
+        ms = np.zeros(64, np.float32)
+        ns, best = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().tpd_batch_autotune(self._ctx, ptr(ms), 64, ctypes.byref(ns), ctypes.byref(best)))
+        return {"sort_ms_per_shape": [round(float(x), 4) for x in ms[: ns.value]], "best_shape": int(best.value)}
+
 
 def sort_pairs_device(engine: GaussianEngine, keys_ptr: int, vals_ptr: int, keys_alt_ptr: int, vals_alt_ptr: int,
                       n: int, key_bits: int, stream: int | None = None) -> bool:
