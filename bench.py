@@ -2,12 +2,20 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-A step is one plain forward frame (GaussianEngine::rasterFrame equivalent: preprocess, scan +
-duplicate, radix sort, ranges, blend, synchronised) of the workload "clusters-1M-1920x1080":
-1M Gaussians in 64 anisotropic clusters (seeded, synthetic), camera at distance 8, SH degree 3,
-16x16 tiles.  `value` is frames (views) per second.  The sort-load variant (log-scale mean -3,
-~10-20M pairs) is run for the sort's HBM fraction.  Stage times come from HIP events recorded on
-the render stream inside the timed frames.
+A step is one batched forward frame of the workload "multiview-32-clusters-1M": one scene of 1M
+Gaussians in 64 anisotropic clusters (seeded, synthetic) seen by 32 cameras on a Fibonacci sphere
+of radius 8 looking at the origin, resolutions drawn (seed 4) from 256x256 ... 1920x1080, SH
+degree 3, 16x16 tiles, all 32 views in one replay of the recorded HIP graph (launch + wait).
+`value` is views per second of that batched frame; the same 32 views rendered as 32 sequential
+plain frames (render + wait each) give the plain rate beside it.
+
+Also measured (each printed in the same JSON line):
+  plain_1080p            the same scene from one camera at distance 8, 1920x1080 (plain frames)
+  sort-load variant      log-scale mean -3 (~18M pairs) for the sort's HBM fraction
+  fwd_bwd_3M_1080p       forward + backward of 3M clustered Gaussians at 1920x1080, dL/dpixel ~ N(0,1)
+  multiscene_16x4_step   16 scenes (log-uniform 50k..2M Gaussians) x 4 views (512..1600 x 512..900),
+                         forward + backward of all 64 views in one recorded graph
+Stage times come from HIP events recorded on the render stream.
 
 One JSON line is printed.  Only one GPU is used; --gpus above 1 exits with a message.
 """
@@ -26,18 +34,100 @@ sys.path.insert(0, ROOT)
 
 METRIC = "views/s batched vs plain; fwd+bwd ms @3M Gaussians 1080p; sort % HBM peak"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 157.3
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def camera_uniforms(eye, w, h):
+def camera_uniforms(eye, w, h, up=(0.0, -1.0, 0.0)):
     from torpedo_amd.engine import PerspectiveCamera
     cam = PerspectiveCamera(w, h)
-    cam.lookAt(eye, [0, 0, 0], [0, -1, 0])
-    view, pv, fo = cam.uniforms()
-    return view, pv, fo
+    cam.lookAt(list(eye), [0, 0, 0], list(up))
+    return cam.uniforms()
+
+
+def med(xs):
+    return float(np.median(np.asarray(xs, np.float64)))
+
+
+def multiview_views(scene_id=0):
+    from torpedo_amd import scenes
+    out = []
+    for eye, (w, h) in scenes.multiview_rig(32, 8.0, seed=4):
+        v, pv, fo = camera_uniforms(eye, w, h, up=(0.0, 0.0, 1.0))
+        out.append((scene_id, v, pv, fo, w, h))
+    return out
+
+
+def onesweep_roofline(P, sort_passes_ms, passes):
+    t_launch = sort_passes_ms / 1000.0 / max(passes, 1)
+    achieved = 24.0 * P / t_launch / 1e9 if t_launch > 0 else 0.0
+    return achieved, t_launch
+
+
+def time_multiview(eng, g, steps, warmup, tune=True):
+    """32 views of one scene: 32 sequential plain frames vs one batched frame."""
+    views = multiview_views()
+    eng.uploadRecords(g, 3)
+    eng.setProfiling(False)
+
+    def plain_pass():
+        t0 = time.perf_counter()
+        for (_, v, pv, fo, w, h) in views:
+            eng.setCamera(v, pv, fo, w, h)
+            eng.render()
+            eng.waitIdle()
+        return time.perf_counter() - t0
+
+    for _ in range(max(warmup, 1)):
+        plain_pass()
+    t_plain = med([plain_pass() for _ in range(max(3, steps // 4))])
+    plain_pairs = 0
+    for (_, v, pv, fo, w, h) in views:
+        eng.setCamera(v, pv, fo, w, h)
+        eng.render()
+        plain_pairs += eng.pairCount
+
+    eng.clearScenes()
+    eng.addScene(g)
+    eng.setViews(views)
+    eng.setBatchGraph(True)
+    eng.rasterBatch()
+    eng.syncBatch()
+    tuned = eng.autotuneBatch() if tune else None
+    for _ in range(warmup):
+        eng.rasterBatch()
+        eng.syncBatch()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.rasterBatch()
+        eng.syncBatch()
+    t_batch = (time.perf_counter() - t0) / steps
+    P = eng.batchPairCount
+    eng.setProfiling(True)
+    gpu = []
+    for _ in range(5):
+        eng.rasterBatch()
+        eng.syncBatch()
+        gpu.append(eng.batchTimes()["frame"])
+    eng.setBatchGraph(False)
+    st = []
+    for _ in range(3):
+        eng.rasterBatch()
+        eng.syncBatch()
+        st.append(eng.batchTimes())
+    eng.setBatchGraph(True)
+    eng.setProfiling(False)
+    stages = {k: round(med([s[k] for s in st]), 4) for k in st[0]}
+    return {"views": len(views), "batched_views_per_s": round(len(views) / t_batch, 2),
+            "plain_views_per_s": round(len(views) / t_plain, 2),
+            "batched_over_plain": round(t_plain / t_batch, 3),
+            "batched_ms_per_frame_wall": round(t_batch * 1000.0, 4), "batched_ms_gpu": round(med(gpu), 4),
+            "plain_ms_32_views_wall": round(t_plain * 1000.0, 4), "pairs": P, "plain_pairs_sum": plain_pairs,
+            "eager_stage_ms": stages, "autotune": tuned,
+            "resolutions": [[v[4], v[5]] for v in views]}
 
 
 def time_plain(eng, g, eye, w, h, steps, warmup, tune=False):
@@ -60,10 +150,10 @@ def time_plain(eng, g, eye, w, h, steps, warmup, tune=False):
     eng.waitIdle()
     dt = time.perf_counter() - t0
     P = eng.pairCount
-    med = {k: float(np.median([s[k] for s in stages])) for k in stages[0]}
+    out = {k: med([s[k] for s in stages]) for k in stages[0]}
     if tuned is not None:
-        med["autotune"] = tuned
-    return dt, P, med
+        out["autotune"] = tuned
+    return dt, P, out
 
 
 def time_fwd_bwd(eng, steps, warmup):
@@ -96,23 +186,80 @@ def time_fwd_bwd(eng, steps, warmup):
     P = eng.pairCount
     for b in bufs + [d_up]:
         eng.free(b)
-    med = {k: round(float(np.median([r[k] for r in rec])), 4) for k in rec[0]}
-    return {"ms": med["fwd_bwd_span"], "wall_ms_per_step": round(wall, 4), "pairs": P, "gaussians": n,
-            "width": W, "height": H, "stage_ms": {k: v for k, v in med.items() if k not in ("radix_passes",)}}
+    eng.setProfiling(False)
+    m = {k: round(med([r[k] for r in rec]), 4) for k in rec[0]}
+    return {"ms": m["fwd_bwd_span"], "wall_ms_per_step": round(wall, 4), "pairs": P, "gaussians": n,
+            "width": W, "height": H, "stage_ms": {k: v for k, v in m.items() if k not in ("radix_passes",)}}
 
 
-def cpu_baseline(g, eye, w, h):
+def time_multiscene(eng, steps, warmup):
+    """16 scenes x 4 views, forward + backward of all 64 views in one recorded graph."""
+    from torpedo_amd import scenes
+    sizes = scenes.multiscene_sizes(16, 5)
+    rigs = scenes.multiscene_views(16, 4, 8.0, 5)
+    eng.clearScenes()
+    for s, n in enumerate(sizes):
+        eng.addScene(scenes.cluster_scene(n, seed=5 + s))
+    views = []
+    for s in range(16):
+        for eye, (w, h) in rigs[s]:
+            v, pv, fo = camera_uniforms(eye, w, h, up=(0.0, 0.0, 1.0))
+            views.append((s, v, pv, fo, w, h))
+    eng.setViews(views)
+    npix = sum(v[4] * v[5] for v in views)
+    dl = np.random.default_rng(5).standard_normal(3 * npix).astype(np.float32)
+    d_dl = eng.alloc(dl.nbytes)
+    eng.upload(d_dl, dl)
+    grads = [[eng.alloc(max(n, 1) * k * 4) for k in (3, 3, 4, 1, 48)] for n in sizes]
+    eng.setBatchGraph(True)
+    eng.setProfiling(False)
+    for _ in range(max(warmup, 1)):
+        eng.batchStep(d_dl, grads)
+        eng.syncBatch()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.batchStep(d_dl, grads)
+        eng.syncBatch()
+    wall = (time.perf_counter() - t0) / steps
+    P = eng.batchPairCount
+    eng.setProfiling(True)
+    gpu = []
+    for _ in range(3):
+        eng.batchStep(d_dl, grads)
+        eng.syncBatch()
+        gpu.append(eng.batchTimes()["frame"])
+    eng.setProfiling(False)
+    for gs in grads:
+        for b in gs:
+            eng.free(b)
+    eng.free(d_dl)
+    eng.clearScenes()
+    return {"scenes": 16, "views": len(views), "gaussians": int(sum(sizes)), "pixels": int(npix), "pairs": P,
+            "ms_per_step_wall": round(wall * 1000.0, 4), "ms_per_step_gpu": round(med(gpu), 4),
+            "views_per_s": round(len(views) / wall, 2)}
+
+
+def cpu_baseline(g, views, budget_s=12.0):
+    """The oracle (multithreaded C restatement) on the first views of the same 32-view workload,
+    until about budget_s seconds of CPU work; views per second."""
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    cam = oracle.make_camera(eye, [0, 0, 0], [0, -1, 0], w, h)
-    oracle.render_timed(g[:1000], cam, 3, nthreads=threads)  # warm-up
+    oracle.render_timed(g[:1000], oracle.make_camera([0, 0, -8], [0, 0, 0], [0, -1, 0], 64, 64), 3,
+                        nthreads=threads)  # warm-up
+    done, pairs, sizes = 0, 0, []
     t0 = time.perf_counter()
-    P, _, _, st = oracle.render_timed(g, cam, 3, nthreads=threads)
+    for (_, v, pv, fo, w, h) in views:
+        cam = oracle.Camera(v, pv, fo, w, h)
+        P, _, _, _ = oracle.render_timed(g, cam, 3, nthreads=threads)
+        done += 1
+        pairs += P
+        sizes.append(f"{w}x{h}")
+        if time.perf_counter() - t0 > budget_s:
+            break
     dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "views/s", "cores": threads, "kind": "port",
-            "sample": f"1 frame of the same workload (P={P}); stage s: "
-                      + ", ".join(f"{k}={v:.3f}" for k, v in zip(
-                          ["preprocess", "scan+dup", "sort", "ranges", "blend"], st))}
+    return {"value": round(done / dt, 4), "unit": "views/s", "cores": threads, "kind": "port",
+            "sample": f"the first {done} of the 32 views ({', '.join(sizes)}), {pairs} pairs, rendered one "
+                      f"after another by the oracle's C restatement with {threads} threads"}
 
 
 def main():
@@ -123,8 +270,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
-    ap.add_argument("--only-sort-load", action="store_true",
-                    help="run only the sort-load frames (for profiler runs of the radix pass)")
+    ap.add_argument("--only", choices=["multiview", "sort-load", "fwd-bwd", "multiscene"], default=None,
+                    help="run one leg only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); views and scenes would shard "
@@ -140,82 +287,106 @@ def main():
     eng = GaussianEngine(device=0)
     if not tune:
         eng.setSortShape(args.sort_shape)
-    if args.only_sort_load:
+    if args.only == "sort-load":
         g2 = scenes.cluster_scene(1_000_000, seed=2, log_scale_mean=-3.0)
         dt2, P2, st2 = time_plain(eng, g2, eye, W, H, args.steps, args.warmup, tune=tune)
-        t_launch = st2["sort_passes"] / 1000.0 / st2["radix_passes"]
-        # A plain copy of 256 MiB (read + write = 512 MiB per launch) for counter calibration.
-        copy_gbs = eng.measureCopy(256 << 20, 3)
+        ach, t_launch = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
+        copy_gbs = eng.measureCopy(256 << 20, 3)  # a plain copy of 256 MiB for counter calibration
         print(json.dumps({"workload": "clusters-1M-1920x1080 log-scale mean -3", "pairs": P2, "stage_ms": st2,
-                          "onesweep_avg_launch_us": round(t_launch * 1e6, 2),
-                          "onesweep_gbs": round(24.0 * P2 / t_launch / 1e9, 1),
+                          "onesweep_avg_launch_us": round(t_launch * 1e6, 2), "onesweep_gbs": round(ach, 1),
                           "copy_256MiB_gbs": round(copy_gbs, 1)}), flush=True)
         eng.close()
         return
+    if args.only == "fwd-bwd":
+        print(json.dumps(time_fwd_bwd(eng, args.steps, args.warmup)), flush=True)
+        eng.close()
+        return
+    if args.only == "multiscene":
+        print(json.dumps(time_multiscene(eng, args.steps, args.warmup)), flush=True)
+        eng.close()
+        return
+
     log("[bench] generating clusters-1M scene")
     g = scenes.cluster_scene(1_000_000, seed=2)
-    log("[bench] timing plain forward frames")
+    log("[bench] 32-view batched vs plain")
+    mv = time_multiview(eng, g, args.steps, args.warmup, tune=tune)
+    log(f"[bench] multiview: {mv}")
+    if args.only == "multiview":
+        print(json.dumps(mv), flush=True)
+        eng.close()
+        return
+
+    log("[bench] plain 1080p frames")
     dt, P, st = time_plain(eng, g, eye, W, H, args.steps, args.warmup, tune=tune)
-    ms = 1000.0 * dt / args.steps
-    log(f"[bench] plain: {ms:.3f} ms/frame, P={P}, stages {st}")
+    ms_plain = 1000.0 * dt / args.steps
+    log(f"[bench] plain: {ms_plain:.3f} ms/frame, P={P}, stages {st}")
 
     log("[bench] sort-load variant (log-scale mean -3)")
     g2 = scenes.cluster_scene(1_000_000, seed=2, log_scale_mean=-3.0)
-    dt2, P2, st2 = time_plain(eng, g2, eye, W, H, max(5, args.steps // 2), 2, tune=tune)
+    _, P2, st2 = time_plain(eng, g2, eye, W, H, max(5, args.steps // 2), 2, tune=tune)
+    del g2
     log(f"[bench] sort-load: P={P2}, stages {st2}")
 
-    # Roofline of the radix pass (onesweep) kernel: 24 algorithmic bytes per pair per pass
-    # (key + value read and written), average launch = all passes / pass count.
-    passes = st2["radix_passes"]
-    t_launch_s = st2["sort_passes"] / 1000.0 / passes
-    achieved = 24.0 * P2 / t_launch_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_onesweep", "pairs": P2, "passes": passes,
-                "avg_launch_us": round(t_launch_s * 1e6, 2)}
-    t_launch_c2 = st["sort_passes"] / 1000.0 / max(st["radix_passes"], 1)
-    sort_frac_c2 = 24.0 * P / t_launch_c2 / 1e9 / HBM_PEAK_GBS if P else 0.0
+    # Roofline of the radix pass (k_onesweep) in the batched frame: 24 algorithmic bytes per pair
+    # per pass (key + value read and written); average launch = radix-pass time / passes.
+    es = mv["eager_stage_ms"]
+    ach, t_launch = onesweep_roofline(mv["pairs"], es["sort_passes"], es["radix_passes"])
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_onesweep",
+                "pairs": mv["pairs"], "passes": es["radix_passes"], "avg_launch_us": round(t_launch * 1e6, 2),
+                "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
+    ach2, t2 = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
+    ach1, _ = onesweep_roofline(P, st["sort_passes"], st["radix_passes"])
+
     log("[bench] forward + backward, 3M Gaussians 1920x1080")
     fb = time_fwd_bwd(eng, max(5, args.steps // 2), 2)
     log(f"[bench] fwd+bwd: {fb}")
+    log("[bench] 16 scenes x 4 views training step")
+    ms5 = time_multiscene(eng, max(5, args.steps // 2), 2)
+    log(f"[bench] multiscene: {ms5}")
     copy_gbs = eng.measureCopy(1 << 30, 10)
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
     # Blend: ~22 FP32 operations + 1 exp per evaluated pixel-splat pair (device-counted).
     ev = st["blend_evaluations"]
     blend_tflops = 22.0 * ev / (st["blend"] / 1000.0) / 1e12
-    roofline_blend = {"bound": "valu", "achieved": round(blend_tflops, 3), "peak": 157.3, "unit": "TFLOP/s",
-                      "frac": round(blend_tflops / 157.3, 4), "evaluations": ev, "kernel": "k_blend",
+    roofline_blend = {"bound": "valu", "achieved": round(blend_tflops, 3), "peak": VALU_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": round(blend_tflops / VALU_PEAK_TFLOPS, 4), "evaluations": ev,
+                      "kernel": "k_blend", "workload": "clusters-1M-1920x1080 plain frame",
                       "note": "22 flop per evaluated pixel-splat pair (plus one exp), vector FP32 peak"}
 
     out = {
         "metric": METRIC,
-        "value": round(args.steps / dt, 3),
+        "value": mv["batched_views_per_s"],
         "unit": "views/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": mv["batched_ms_per_frame_wall"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded SplitMix64 scene, no dataset)",
-        "config": {"workload": "clusters-1M-1920x1080 plain forward", "gaussians": 1_000_000, "width": W,
-                   "height": H, "sh_degree": 3, "tile": 16, "pairs": P, "sort_digit_bits": 8,
-                   "parallelism": "single GPU"},
-        "stage_ms": {k: round(v, 4) for k, v in st.items()
-                     if k not in ("radix_passes", "autotune", "blend_evaluations")},
-        "autotune": {"clusters_1M": st.get("autotune"), "sort_load_variant": st2.get("autotune")},
-        "sort_hbm_frac": {"sort_load_variant": round(achieved / HBM_PEAK_GBS, 4), "pairs": P2,
-                          "clusters_1M": round(sort_frac_c2, 4), "pairs_clusters_1M": P},
+        "data": "synthetic (seeded SplitMix64 scenes and camera rigs, no dataset)",
+        "config": {"workload": "multiview-32-clusters-1M", "gaussians": 1_000_000, "views": 32,
+                   "radius": 8.0, "sh_degree": 3, "tile": 16, "sort_digit_bits": 8, "pairs": mv["pairs"],
+                   "parallelism": "single GPU", "graph": "one recorded HIP graph per frame"},
+        "multiview_32": mv,
+        "plain_1080p": {"ms_per_frame": round(ms_plain, 4), "views_per_s": round(1000.0 / ms_plain, 2), "pairs": P,
+                        "stage_ms": {k: round(v, 4) for k, v in st.items()
+                                     if k not in ("radix_passes", "autotune", "blend_evaluations")},
+                        "autotune": st.get("autotune")},
+        "sort_hbm_frac": {"multiview_32": round(ach / HBM_PEAK_GBS, 4), "pairs_multiview_32": mv["pairs"],
+                          "sort_load_variant": round(ach2 / HBM_PEAK_GBS, 4), "pairs_sort_load": P2,
+                          "plain_1080p": round(ach1 / HBM_PEAK_GBS, 4), "pairs_plain_1080p": P},
         "roofline": roofline,
         "roofline_blend": roofline_blend,
         "fwd_bwd_3M_1080p": fb,
+        "multiscene_16x4_step": ms5,
         "copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle restatement, one frame)")
-        out["cpu_baseline"] = cpu_baseline(g, eye, W, H)
+        log("[bench] CPU baseline (oracle restatement)")
+        out["cpu_baseline"] = cpu_baseline(g, multiview_views())
     eng.close()
     print(json.dumps(out), flush=True)
 

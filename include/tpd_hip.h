@@ -218,9 +218,10 @@ int tpd_batch_read_image_rgba8(tpd_ctx* ctx, uint32_t view, uint8_t* host);
 int tpd_batch_device_images(tpd_ctx* ctx, float** d_images, uint64_t* total_pixels);
 /* The sorted global (key, value) pairs of the last batch frame; *P receives the count. */
 int tpd_batch_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capacity, uint32_t* P);
-/* With tpd_set_profiling on: ms[0] = GPU time of the last batch frame; for frames issued without the
- * graph also ms[1..5] = count pass, preprocess + duplicate, sort, ranges, blend (else -1). */
-int tpd_batch_get_times(tpd_ctx* ctx, float* ms);
+/* With tpd_set_profiling on: ms[0] = GPU time of the last batch frame; for forward frames issued
+ * without the graph also ms[1..6] = count pass, preprocess + duplicate, sort histograms, radix
+ * passes, ranges, blend (else -1).  *passes receives the radix passes per frame. */
+int tpd_batch_get_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
 /* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
 int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
 

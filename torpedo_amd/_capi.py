@@ -88,7 +88,7 @@ SIGNATURES = {
     "tpd_batch_read_image_rgba8": (_i32, [_vp, _u32, _vp]),
     "tpd_batch_device_images": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
     "tpd_batch_read_pairs": (_i32, [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32)]),
-    "tpd_batch_get_times": (_i32, [_vp, _vp]),
+    "tpd_batch_get_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     "tpd_batch_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_look_at": (None, [_vp, _vp, _vp, _vp]),
     "tpd_perspective": (None, [_u32, _u32, _f32, _f32, _f32, _vp]),

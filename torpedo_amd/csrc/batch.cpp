@@ -89,8 +89,9 @@ struct BatchState {
     bool rendered = false;      // last frame validated
     bool kept_bwd_state = false;
     uint32_t last_pairs = 0;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};  // eager stage events [0..6], frame bracket [7], [8]
     bool timed = false;
+    uint32_t passes = 0;    // radix passes of the recorded frame
 };
 
 static void destroy_graphs(BatchState* b) {
@@ -360,16 +361,19 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     if (events) hipEventRecord(b->ev[1], s);
     launch_batch_preprocess(a, s);
     if (events) hipEventRecord(b->ev[2], s);
+    const uint32_t key_bits = batch_key_bits(b);
+    b->passes = sort_passes(key_bits);
     const bool in_alt = launch_sort_pairs(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
-                                          &b->d_counters[kCtrPairs], b->cap, batch_key_bits(b), s, nullptr, 31);
+                                          &b->d_counters[kCtrPairs], b->cap, key_bits, s, events ? b->ev[3] : nullptr,
+                                          31);
     b->sorted_keys = in_alt ? b->d_keys_alt : b->d_keys;
     b->sorted_vals = in_alt ? b->d_vals_alt : b->d_vals;
-    if (events) hipEventRecord(b->ev[3], s);
+    if (events) hipEventRecord(b->ev[4], s);
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc, b->nblocks,
                   &b->d_counters[kCtrScanTicket], s, b->cap);
-    if (events) hipEventRecord(b->ev[4], s);
-    launch_batch_blend(blend_args(b), s);
     if (events) hipEventRecord(b->ev[5], s);
+    launch_batch_blend(blend_args(b), s);
+    if (events) hipEventRecord(b->ev[6], s);
     (void)hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost, s);
 }
 
@@ -435,7 +439,7 @@ static int launch_frame(tpd_ctx* c, BatchState* b, int mode, const float* dL, ui
     if (c->profile)
         for (auto& ev : b->ev)
             if (!ev && (e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-    if (c->profile) hipEventRecord(b->ev[6], s);
+    if (c->profile) hipEventRecord(b->ev[7], s);
     if (b->use_graph) {
         if (!b->exec[mode]) {
             hipGraph_t g = nullptr;
@@ -461,7 +465,7 @@ static int launch_frame(tpd_ctx* c, BatchState* b, int mode, const float* dL, ui
         if (mode == 2) record_backward(b, dL, accumulate, s);
     }
     if (c->profile) {
-        hipEventRecord(b->ev[7], s);
+        hipEventRecord(b->ev[8], s);
         b->timed = true;
     }
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "tpd_batch: launch");
@@ -774,18 +778,20 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_read_pairs");
 }
 
-// ms[0] = whole frame (first launch to last), and for eager profiled frames ms[1..5] = count
-// pass (with entity matrices), preprocess + duplicate, sort, ranges, blend.
-int tpd_batch_get_times(tpd_ctx* c, float* ms) {
+// ms[0] = whole frame (first launch to last); for eager profiled forward frames ms[1..6] = count
+// pass (with entity matrices), preprocess + duplicate, sort histograms, radix passes, ranges,
+// blend (else -1).  *passes = radix passes per frame.
+int tpd_batch_get_times(tpd_ctx* c, float* ms, uint32_t* passes) {
     if (!c || !ms) return fail(TPD_ERR_INVALID, "tpd_batch_get_times: bad argument");
     hipSetDevice(c->device);
     BatchState* b = state(c);
     if (!c->profile || !b->timed) return fail(TPD_ERR_STATE, "tpd_batch_get_times: no profiled batch frame");
-    hipError_t e = hipEventSynchronize(b->ev[7]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], b->ev[6], b->ev[7]);
-    for (int i = 1; i <= 5; ++i) ms[i] = -1.0f;
-    if (e == hipSuccess && !b->use_graph)
-        for (int i = 1; i <= 5 && e == hipSuccess; ++i) e = hipEventElapsedTime(&ms[i], b->ev[i - 1], b->ev[i]);
+    hipError_t e = hipEventSynchronize(b->ev[8]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], b->ev[7], b->ev[8]);
+    for (int i = 1; i <= 6; ++i) ms[i] = -1.0f;
+    if (e == hipSuccess && !b->use_graph && b->pending_mode == 0)
+        for (int i = 1; i <= 6 && e == hipSuccess; ++i) e = hipEventElapsedTime(&ms[i], b->ev[i - 1], b->ev[i]);
+    if (passes) *passes = b->passes;
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_get_times");
 }
 
@@ -811,7 +817,7 @@ int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* 
             if ((rc = launch_frame(c, b, 0, nullptr, 0, s)) != TPD_OK) break;
             if ((rc = validate(c, b)) != TPD_OK) break;
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, b->ev[2], b->ev[3]) != hipSuccess) {
+            if (hipEventElapsedTime(&ms, b->ev[2], b->ev[4]) != hipSuccess) {
                 rc = fail(TPD_ERR_HIP, "tpd_batch_autotune: timing");
                 break;
             }

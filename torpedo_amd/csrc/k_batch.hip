@@ -47,7 +47,13 @@ __device__ __forceinline__ bool project_in_view(const float* rec, const EntityMa
 
 // ------------------------------------------------------------------ count pass
 
-__global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a) {
+// The read-only tables come as separate __restrict__ parameters so that the compiler may fetch
+// them with scalar loads (uniform addresses, no aliasing stores).
+__global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
+                                                           const BatchViewDev* __restrict__ views,
+                                                           const uint32_t* __restrict__ scene_views,
+                                                           const uint32_t* __restrict__ block_scene,
+                                                           const EntityMats* __restrict__ mats) {
     constexpr uint32_t kWaves = kPreBlock / kWave;
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_w[kWaves][kWave];  // wave sums of the current chunk of 64 views
@@ -61,26 +67,40 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a) {
         a.counters[kCtrOverflow] = 0;
     }
     __syncthreads();
-    const uint32_t b = s_part;
+    // The ticket is block-uniform: read it into a scalar register so that the scene and view
+    // tables below are fetched with scalar loads.
+    const uint32_t b = __builtin_amdgcn_readfirstlane(s_part);
     if (b >= a.nblocks) return;  // tickets are reset every frame; true only without Gaussians
-    const BatchSceneDev& sc = a.scenes[a.block_scene[b]];
+    const BatchSceneDev& sc = scenes[block_scene[b]];
     const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
     const uint32_t g = lb * kPreBlock + tid;
     const bool valid = g < sc.n;
     const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
-    const float* rec = sc.gauss + (size_t)(valid ? g : 0u) * 60;
     const uint32_t entry0 = sc.entry_base + lb * nviews;
+    // Only position, opacity, quaternion and scale decide the tile count; the colour part of the
+    // projection is unused here.
+    float rec[60];
+#pragma unroll
+    for (int q = 0; q < 60; ++q) rec[q] = 0.0f;
+    if (valid) {
+        const float4* r4 = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float4 t = r4[q];
+            rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
+        }
+    }
 
     // Per view: the block's pair count, stored in bases[] until the look-back gives positions.
     uint32_t my_total = 0;  // threads 0..63: sum of the counts they stored
     for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
         const uint32_t kn = min(kWave, nviews - k0);
         for (uint32_t kk = 0; kk < kn; ++kk) {
-            const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k0 + kk]];
+            const BatchViewDev& vw = views[scene_views[sc.view_begin + k0 + kk]];
             uint32_t t = 0;
             if (valid) {
                 Projected p;
-                project_in_view(rec, a.mats[vw.mat_base + e], vw, sc.sh_degree, p);
+                project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
                 t = p.tiles;
             }
             const uint32_t ws = wave_incl_scan(t);
@@ -127,12 +147,18 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a) {
 
 void launch_batch_count(const BatchArgs& a, hipStream_t s) {
     const uint32_t nb = a.nblocks ? a.nblocks : 1u;  // one block clears the ranges of an empty batch
-    hipLaunchKernelGGL(k_batch_count, dim3(nb), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_count, dim3(nb), dim3(kPreBlock), 0, s, a, a.scenes, a.views, a.scene_views,
+                       a.block_scene, a.mats);
 }
 
 // ------------------------------------------------------------------ preprocess + duplicate
 
-__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs a) {
+__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs a,
+                                                                const BatchSceneDev* __restrict__ scenes,
+                                                                const BatchViewDev* __restrict__ views,
+                                                                const uint32_t* __restrict__ scene_views,
+                                                                const uint32_t* __restrict__ block_scene,
+                                                                const EntityMats* __restrict__ mats) {
     __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
     __shared__ uint32_t s_incl[kPreBlock];
     __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
@@ -140,7 +166,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
 
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     const uint32_t b = blockIdx.x;
-    const BatchSceneDev& sc = a.scenes[a.block_scene[b]];
+    const BatchSceneDev& sc = scenes[block_scene[b]];
     const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
     const uint32_t g = lb * kPreBlock + tid;
     const bool valid = g < sc.n;
@@ -163,12 +189,12 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
     }
 
     for (uint32_t k = 0; k < nviews; ++k) {
-        const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k]];
+        const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
         Projected p;
         p.tiles = 0;
         if (valid) {
-            if (project_in_view(rec, a.mats[vw.mat_base + e], vw, sh_degree, p)) {
+            if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) {
                 a.xyzr[item] = make_float4(p.px, p.py, p.vz, p.radius);
                 a.conop[item] = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
                 a.rgb[item] = make_float4(p.r, p.g, p.b, 0.0f);
@@ -205,50 +231,62 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
 
 void launch_batch_preprocess(const BatchArgs& a, hipStream_t s) {
     if (a.nblocks == 0) return;
-    hipLaunchKernelGGL(k_batch_preprocess, dim3(a.nblocks), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_preprocess, dim3(a.nblocks), dim3(kPreBlock), 0, s, a, a.scenes, a.views,
+                       a.scene_views, a.block_scene, a.mats);
 }
 
 // ------------------------------------------------------------------ blend, forward and backward
 
-__device__ __forceinline__ TileView batch_tile_view(const BatchBlendArgs& a, uint32_t tile) {
-    const BatchViewDev& vw = a.views[a.tile_view[tile]];
+__device__ __forceinline__ TileView batch_tile_view(const BatchViewDev* __restrict__ views,
+                                                    const uint32_t* __restrict__ tile_view, uint32_t tile) {
+    const BatchViewDev& vw = views[tile_view[tile]];
     const uint32_t local = tile - vw.tile_base;
     return TileView{local % vw.gx, local / vw.gx, vw.cam.width, vw.cam.height, vw.gx, vw.gy, vw.pix_base};
 }
 
-__global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArgs a) {
+__global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArgs a,
+                                                             const BatchViewDev* __restrict__ views,
+                                                             const uint32_t* __restrict__ tile_view) {
     __shared__ BlendLds lds;
     if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const TileView tv = batch_tile_view(a, tile);
+    const TileView tv = batch_tile_view(views, tile_view, tile);
     blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.image, a.rgba8, nullptr);
 }
 
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s) {
     if (a.ntiles == 0) return;
-    hipLaunchKernelGGL(k_batch_blend, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+    hipLaunchKernelGGL(k_batch_blend, dim3(a.ntiles), dim3(kTilePixels), 0, s, a, a.views, a.tile_view);
 }
 
-__global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlendArgs a) {
+__global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlendArgs a,
+                                                                 const BatchViewDev* __restrict__ views,
+                                                                 const uint32_t* __restrict__ tile_view) {
     __shared__ BlendBwdLds lds;
     if (a.counters[kCtrOverflow]) return;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    const TileView tv = batch_tile_view(a, tile);
+    const TileView tv = batch_tile_view(views, tile_view, tile);
     blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.dL_dimg, a.offsets, a.partials);
 }
 
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s) {
     if (a.ntiles == 0) return;
-    hipLaunchKernelGGL(k_batch_blend_bwd, dim3(a.ntiles), dim3(kTilePixels), 0, s, a);
+    hipLaunchKernelGGL(k_batch_blend_bwd, dim3(a.ntiles), dim3(kTilePixels), 0, s, a, a.views, a.tile_view);
 }
 
 // ------------------------------------------------------------------ preprocess backward
 
-__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchPreBwdArgs a) {
+__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchPreBwdArgs a,
+                                                                    const BatchSceneDev* __restrict__ scenes,
+                                                                    const BatchViewDev* __restrict__ views,
+                                                                    const uint32_t* __restrict__ scene_views,
+                                                                    const uint32_t* __restrict__ block_scene,
+                                                                    const EntityMats* __restrict__ mats,
+                                                                    const BatchGradsDev* __restrict__ grads) {
     if (a.counters[kCtrOverflow]) return;  // gradients are left untouched; the step is run again
     const uint32_t b = blockIdx.x;
-    const uint32_t si = a.block_scene[b];
-    const BatchSceneDev& sc = a.scenes[si];
+    const uint32_t si = block_scene[b];
+    const BatchSceneDev& sc = scenes[si];
     const uint32_t g = (b - sc.first_block) * kPreBlock + threadIdx.x;
     if (g >= sc.n) return;
     const float* rec = sc.gauss + (size_t)g * 60;
@@ -258,7 +296,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
 #pragma unroll
     for (int k = 0; k < 48; ++k) gsh[k] = 0.f;
     for (uint32_t k = 0; k < sc.nviews; ++k) {
-        const BatchViewDev& vw = a.views[a.scene_views[sc.view_begin + k]];
+        const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
         const uint32_t tiles = a.tiles[item];
         if (tiles == 0) continue;
@@ -269,17 +307,18 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
         for (uint32_t t = 0; t < tiles; ++t)
 #pragma unroll
             for (int q = 0; q < 9; ++q) gr[q] += src[t * 9 + q];
-        const EntityMats& m = a.mats[vw.mat_base + e];
+        const EntityMats& m = mats[vw.mat_base + e];
         accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, gm, gls, gq, gop, gsh);
     }
-    const BatchGradsDev& gd = a.grads[si];
+    const BatchGradsDev& gd = grads[si];
     store_param_grads(g, a.accumulate != 0, gm, gls, gq, gop, gsh, gd.means, gd.log_scales, gd.quats,
                       gd.opacity_logits, gd.sh);
 }
 
 void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s) {
     if (a.nblocks == 0) return;
-    hipLaunchKernelGGL(k_batch_preprocess_bwd, dim3(a.nblocks), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_preprocess_bwd, dim3(a.nblocks), dim3(kPreBlock), 0, s, a, a.scenes, a.views,
+                       a.scene_views, a.block_scene, a.mats, a.grads);
 }
 
 }  // namespace tpd

@@ -548,12 +548,16 @@ class GaussianEngine:
         check(lib().tpd_batch_read_pairs(self._ctx, ptr(k), ptr(v), max(P, 1), ctypes.byref(n)))
         return k[:P], v[:P]
 
-    BATCH_STAGES = ("frame", "count", "preprocess_duplicate", "sort", "ranges", "blend")
+    BATCH_STAGES = ("frame", "count", "preprocess_duplicate", "sort_histogram", "sort_passes", "ranges", "blend")
 
     def batchTimes(self) -> dict:
-        ms = np.zeros(6, np.float32)
-        check(lib().tpd_batch_get_times(self._ctx, ptr(ms)))
-        return {k: float(v) for k, v in zip(self.BATCH_STAGES, ms) if v >= 0}
+        """GPU milliseconds of the last profiled batch frame (stages only for eager forward frames)."""
+        ms = np.zeros(7, np.float32)
+        passes = ctypes.c_uint32()
+        check(lib().tpd_batch_get_times(self._ctx, ptr(ms), ctypes.byref(passes)))
+        out = {k: float(v) for k, v in zip(self.BATCH_STAGES, ms) if v >= 0}
+        out["radix_passes"] = int(passes.value)
+        return out
 
     def autotuneBatch(self) -> dict:
         ms = np.zeros(64, np.float32)

@@ -18,6 +18,8 @@ opacity = sigmoid(N(0,1)), sh DC ~ N(0.5,0.2), other sh ~ N(0,0.2).
 from __future__ import annotations
 
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -25,6 +27,19 @@ GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
 _M2 = np.uint64(0x94D049BB133111EB)
 _CHUNK = 1 << 22
+_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _parallel(fn, n: int, chunk: int) -> None:
+    """Runs fn(lo, hi) over [0, n) in chunks on a thread pool (numpy releases the GIL); the
+    chunking changes no value."""
+    spans = [(lo, min(n, lo + chunk)) for lo in range(0, n, chunk)]
+    if len(spans) <= 1 or _THREADS == 1:
+        for lo, hi in spans:
+            fn(lo, hi)
+        return
+    with ThreadPoolExecutor(_THREADS) as ex:
+        list(ex.map(lambda sp: fn(*sp), spans))
 
 
 def _mix(z: np.ndarray) -> np.ndarray:
@@ -44,12 +59,14 @@ def uniforms(seed: int, attr: int, n: int) -> np.ndarray:
     """n doubles in [0, 1) from SplitMix64 stream (seed, attr): u_i = mix(base + (i+1)*golden) >> 11 * 2^-53."""
     base = _stream_base(seed, attr)
     out = np.empty(n, np.float64)
-    with np.errstate(over="ignore"):
-        for lo in range(0, n, _CHUNK):
-            hi = min(n, lo + _CHUNK)
+
+    def run(lo, hi):
+        with np.errstate(over="ignore"):
             i = np.arange(lo + 1, hi + 1, dtype=np.uint64)
             z = _mix(base + i * GOLDEN)
             out[lo:hi] = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    _parallel(run, n, _CHUNK)
     return out
 
 
@@ -57,12 +74,16 @@ def normals(seed: int, attr: int, n: int) -> np.ndarray:
     """n standard normals by Box-Muller over pairs of uniforms of stream (seed, attr)."""
     m = (n + 1) // 2
     u = uniforms(seed, attr, 2 * m)
-    u1 = 1.0 - u[0::2]
-    u2 = u[1::2]
-    r = np.sqrt(-2.0 * np.log(u1))
     out = np.empty(2 * m, np.float64)
-    out[0::2] = r * np.cos(2.0 * math.pi * u2)
-    out[1::2] = r * np.sin(2.0 * math.pi * u2)
+
+    def run(lo, hi):  # pairs [lo, hi)
+        u1 = 1.0 - u[2 * lo:2 * hi:2]
+        u2 = u[2 * lo + 1:2 * hi:2]
+        r = np.sqrt(-2.0 * np.log(u1))
+        out[2 * lo:2 * hi:2] = r * np.cos(2.0 * math.pi * u2)
+        out[2 * lo + 1:2 * hi:2] = r * np.sin(2.0 * math.pi * u2)
+
+    _parallel(run, m, _CHUNK // 2)
     return out[:n]
 
 
@@ -93,10 +114,12 @@ def attributes(seed: int, n: int, log_scale_mean: float = -4.5, out: np.ndarray 
     g[:, 8:11] = np.exp(log_scale_mean + 0.5 * normals(seed, 4, 3 * n)).reshape(n, 3).astype(np.float32)
     g[:, 11] = 1.0
     g[:, 12:15] = (0.5 + 0.2 * normals(seed, 5, 3 * n)).reshape(n, 3).astype(np.float32)
-    for lo in range(0, n, _CHUNK // 16):
-        hi = min(n, lo + _CHUNK // 16)
+
+    def sh_rest(lo, hi):
         g[lo:hi, 15:60] = (0.2 * normals(seed * 1000003 + lo, 6, 45 * (hi - lo))).reshape(hi - lo, 45).astype(
             np.float32)
+
+    _parallel(sh_rest, n, _CHUNK // 16)
     return g
 
 
@@ -139,6 +162,18 @@ def multiview_rig(count: int = 32, radius: float = 8.0, seed: int = 4):
     eyes = fibonacci_sphere(count, radius, seed)
     pick = np.minimum((uniforms(seed, 20, count) * len(RESOLUTIONS)).astype(np.int64), len(RESOLUTIONS) - 1)
     return [(eyes[k], RESOLUTIONS[pick[k]]) for k in range(count)]
+
+
+def multiscene_views(count: int = 16, per_scene: int = 4, radius: float = 8.0, seed: int = 5):
+    """Per scene s: per_scene (eye, (w, h)) with eyes on a Fibonacci sphere (seed + s) and
+    W ~ U{512..1600}, H ~ U{512..900}."""
+    out = []
+    for s in range(count):
+        eyes = fibonacci_sphere(per_scene, radius, seed + s)
+        uw, uh = uniforms(seed + s, 40, per_scene), uniforms(seed + s, 41, per_scene)
+        out.append([(eyes[k], (512 + min(int(uw[k] * 1089), 1088), 512 + min(int(uh[k] * 389), 388)))
+                    for k in range(per_scene)])
+    return out
 
 
 def multiscene_sizes(count: int = 16, seed: int = 5):
