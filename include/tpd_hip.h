@@ -225,6 +225,15 @@ int tpd_batch_get_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
 /* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
 int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
 
+/* ----- point-cloud files -----
+ * GaussianPoint::fromModel (volumetric/src/GaussianGeometry.cpp:59-127): reads a 3DGS PLY file (ascii or
+ * binary, any PLY property types) into *n GaussianPoint records (60 floats each) allocated with malloc;
+ * release them with tpd_free_host.  Opacity logits become sigmoid(logit), rot_0..3 (w first) become a
+ * normalised (x, y, z, w) quaternion, log-scales become exp(scale) with modifier 1, and the SH features
+ * are f_dc_0..2 followed by the f_rest_* properties in file order (missing ones zero). */
+int tpd_load_ply(const char* path, float** records, uint32_t* n);
+void tpd_free_host(void* p);
+
 /* ----- host camera math (torpedo/math, rendering/src/Camera.cpp, extension/src/PerspectiveCamera.cpp) ----- */
 /* Camera::lookAt (Camera.cpp:3-16) */
 void tpd_look_at(const float eye[3], const float center[3], const float up[3], float view[16]);

@@ -34,6 +34,7 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     if os.path.isdir("/root/reference/torpedo"):
         subprocess.run([os.path.join(_HERE, "ref_camera.sh")], check=True, stdout=subprocess.DEVNULL)
+        subprocess.run([os.path.join(_HERE, "ref_ply.sh")], check=True, stdout=subprocess.DEVNULL)
 
 
 def lib():
@@ -70,6 +71,7 @@ def lib():
         L.tpo_render_f32.restype = ctypes.c_uint64
         L.tpo_render_f32.argtypes = [_f32p, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _f32p, _f32p, _f32p,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _f64p, ctypes.c_int]
+        L.tpo_gaussian_from_ply.argtypes = [_f32p, ctypes.c_float, _f32p, _f32p, _f32p, ctypes.c_uint32, _f32p]
         L.tpo_render_f64.restype = ctypes.c_uint64
         L.tpo_render_f64.argtypes = [_f32p, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _f32p, _f32p, _f32p,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f64p, ctypes.c_int]
@@ -92,6 +94,44 @@ def ref_camera():
         R.tpr_cross3.argtypes = [_f32p, _f32p, _f32p]
         _REF = R
     return _REF
+
+
+def ref_ply():
+    """The reference's own PLY loader (oracle/_ref/libtpd_ref_ply.so) or None if not built."""
+    path = os.path.join(_HERE, "_ref", "libtpd_ref_ply.so")
+    if not os.path.exists(path):
+        return None
+    R = ctypes.CDLL(path)
+    R.tpr_from_model.restype = ctypes.c_long
+    R.tpr_from_model.argtypes = [ctypes.c_char_p, _f32p, ctypes.c_ulong]
+    return R
+
+
+def ref_from_model(path: str):
+    """(N, 60) records from the reference's GaussianPoint::fromModel, or None when it throws or is not built."""
+    R = ref_ply()
+    if R is None:
+        return None
+    n = R.tpr_from_model(path.encode(), np.zeros(60, np.float32), 0)
+    if n < 0:
+        return None
+    out = np.zeros((max(n, 1), 60), np.float32)
+    R.tpr_from_model(path.encode(), out, n)
+    return out[:n]
+
+
+def gaussians_from_ply_columns(xyz, opacity, rot_wxyz, log_scale, features):
+    """The oracle's restatement of fromModel's per-vertex conversion over column arrays (N x ...)."""
+    L = lib()
+    xyz, rot, ls, ft = _f32(xyz), _f32(rot_wxyz), _f32(log_scale), _f32(features)
+    op = _f32(opacity)
+    n = xyz.shape[0]
+    out = np.zeros((n, 60), np.float32)
+    row = np.zeros(60, np.float32)
+    for i in range(n):
+        L.tpo_gaussian_from_ply(xyz[i], float(op[i]), rot[i], ls[i], np.ascontiguousarray(ft[i]), ft.shape[1], row)
+        out[i] = row
+    return out
 
 
 def _ptr(a):

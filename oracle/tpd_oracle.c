@@ -147,6 +147,22 @@ static void normalize3(const float* v, float* o) {
     o[0] = v[0] * inv; o[1] = v[1] * inv; o[2] = v[2] * inv;
 }
 
+/* GaussianPoint::fromModel's conversion of one PLY vertex (volumetric/src/GaussianGeometry.cpp:108-119):
+ * sigmoid opacity, (rot_1, rot_2, rot_3, rot_0) normalised by v * (1 / sqrt(compensated dot))
+ * (math/vec4.h:248-275), exp scales with modifier 1, nfeat SH features copied (rest zero). */
+void tpo_gaussian_from_ply(const float* xyz, float opacity_logit, const float* rot_wxyz, const float* log_scale,
+                           const float* features, uint32_t nfeat, float* out) {
+    memset(out, 0, 60 * sizeof(float));
+    out[0] = xyz[0]; out[1] = xyz[1]; out[2] = xyz[2];
+    out[3] = 1.0f / (1.0f + expf(-opacity_logit));
+    const float q[4] = {rot_wxyz[1], rot_wxyz[2], rot_wxyz[3], rot_wxyz[0]};
+    const float inv = 1.0f / sqrtf(dot_comp(q, q, 4));
+    for (int i = 0; i < 4; ++i) out[4 + i] = q[i] * inv;
+    for (int i = 0; i < 3; ++i) out[8 + i] = expf(log_scale[i]);
+    out[11] = 1.0f;
+    for (uint32_t k = 0; k < nfeat && k < 48; ++k) out[12 + k] = features[k];
+}
+
 void tpo_look_at(const float* eye, const float* center, const float* up, float* view) {
     float d[3] = {center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]};
     float z[3], x[3], y[3], t[3];

@@ -1,0 +1,101 @@
+// C++ caller of libtpd_hip.so through include/torpedo/*.hpp, written the way a caller of the
+// reference's engine is (demo/HelloGaussian/main.cpp): build a Scene of GaussianPoint groups,
+// compile it, rasterFrame a PerspectiveCamera, transform an entity, render a batch.  Writes what
+// it rendered into the directory argv[1]; tests/test_gpu_cpp_api.py checks it against the oracle.
+#include <torpedo/GaussianEngine.hpp>
+
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace {
+template <typename T>
+void dump(const std::string& path, const T* data, size_t count) {
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(data), static_cast<std::streamsize>(count * sizeof(T)));
+}
+template <typename T>
+void dump(const std::string& path, const std::vector<T>& v) {
+    dump(path, v.data(), v.size());
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: engine_api OUTDIR\n");
+        return 2;
+    }
+    const std::string out = argv[1];
+    try {
+        const auto a = tpd::GaussianPoint::random(3000, 1.0f, {0.f, 0.f, 0.f}, 0.01f, 0.05f, 0.2f, 0.9f, 11);
+        const auto b = tpd::GaussianPoint::random(2000, 0.6f, {0.3f, -0.2f, 0.1f}, 0.01f, 0.03f, 0.3f, 1.0f, 12);
+        tpd::GaussianPoint single{};
+        single.position = {0.0f, 0.0f, 0.5f};
+        single.opacity = 0.8f;
+        single.quaternion = {0.0f, 0.0f, 0.0f, 1.0f};
+        single.scale = {0.1f, 0.05f, 0.02f, 1.0f};
+        single.sh = tpd::utils::rgb2sh(0.9f, 0.2f, 0.1f);
+
+        tpd::Scene scene;
+        scene.add(tpd::ent::group(a));
+        const auto eb = scene.add(tpd::ent::group(b));
+        scene.add(single);
+
+        tpd::GaussianEngine engine(320, 200);
+        engine.compile(scene);
+        tpd::PerspectiveCamera cam(320, 200);
+        cam.lookAt({0.2f, -0.1f, -3.0f}, {0.f, 0.f, 0.f}, {0.f, -1.f, 0.f});
+        engine.rasterFrame(cam);
+
+        dump(out + "/records.f32", scene.dataAll<tpd::GaussianPoint>());
+        dump(out + "/view.f32", cam.getViewMatrixData(), 16);
+        dump(out + "/proj.f32", cam.getProjectionData(), 16);
+        dump(out + "/image.f32", engine.readImage());
+        dump(out + "/image.rgba8", engine.readRGBA8());
+        std::vector<uint64_t> keys;
+        std::vector<uint32_t> vals;
+        engine.readPairs(keys, vals);
+        dump(out + "/keys.u64", keys);
+        dump(out + "/vals.u32", vals);
+
+        // TransformHost::transform of the second group, then the same camera again
+        const tpd::mat4 M{0.9f, -0.2f, 0.0f, 0.4f, 0.2f, 0.9f, 0.1f, -0.3f, 0.0f, -0.1f, 1.1f, 0.0f, 0.f, 0.f, 0.f, 1.f};
+        engine.getTransformHost()->transform(eb, M);
+        engine.rasterFrame(cam);
+        dump(out + "/image_transformed.f32", engine.readImage());
+        dump(out + "/model.f32", M.data_ptr(), 16);
+
+        // batch of two views of the (untransformed) scene
+        const uint32_t sid = engine.addScene(scene);
+        tpd::PerspectiveCamera cam2(128, 96);
+        cam2.setVerticalFov(45.0f);
+        cam2.lookAt({-1.0f, 0.5f, -2.5f}, {0.f, 0.f, 0.f}, {0.f, -1.f, 0.f});
+        engine.setViews({{sid, &cam, 320, 200}, {sid, &cam2, 128, 96}});
+        engine.rasterBatch();
+        dump(out + "/batch0.f32", engine.readBatchImage(0));
+        dump(out + "/batch1.f32", engine.readBatchImage(1));
+        dump(out + "/view2.f32", cam2.getViewMatrixData(), 16);
+        dump(out + "/proj2.f32", cam2.getProjectionData(), 16);
+
+        // compile of an empty scene warns and keeps the compiled one (GaussianEngine.cpp:362-365)
+        tpd::Scene empty;
+        engine.compile(empty);
+        engine.rasterFrame(cam);
+        dump(out + "/image_after_empty.f32", engine.readImage());
+
+        // a bad view is reported as std::runtime_error carrying the library's message
+        std::string msg = "no exception";
+        try {
+            engine.setViews({{7, &cam, 10, 10}});
+        } catch (const std::runtime_error& e) {
+            msg = e.what();
+        }
+        std::ofstream(out + "/error.txt") << msg;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "engine_api: %s\n", e.what());
+        return 1;
+    }
+    std::printf("engine_api: ok\n");
+    return 0;
+}

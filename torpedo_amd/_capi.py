@@ -90,6 +90,8 @@ SIGNATURES = {
     "tpd_batch_read_pairs": (_i32, [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32)]),
     "tpd_batch_get_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     "tpd_batch_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "tpd_load_ply": (_i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(_u32)]),
+    "tpd_free_host": (None, [_vp]),
     "tpd_look_at": (None, [_vp, _vp, _vp, _vp]),
     "tpd_perspective": (None, [_u32, _u32, _f32, _f32, _f32, _vp]),
     "tpd_fy_from_fov": (_f32, [_f32]),

@@ -23,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtpd_hip.so")
 
 SOURCES = ["k_forward.hip", "k_sort.hip", "k_backward.hip", "k_batch.hip", "render.cpp", "backward.cpp", "batch.cpp",
-           "tuner.cpp", "k_util.hip", "capi.cpp"]
+           "tuner.cpp", "k_util.hip", "ply.cpp", "capi.cpp"]
 
 
 def hipcc() -> str:
@@ -40,6 +40,23 @@ def command(out: str = LIB) -> list[str]:
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + ["-o", out]
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "engine_api.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "engine_api")
+
+
+def build_cpp_test() -> str:
+    """The C++ API test program (include/torpedo/*.hpp over libtpd_hip.so), linked in-tree."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+           "-L", HERE, "-ltpd_hip", "-Wl,-rpath,$ORIGIN/../../torpedo_amd", "-o", CPP_TEST_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the C++ API test")
+    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
+    return CPP_TEST_BIN
 
 
 def build(verbose: bool = False) -> str:

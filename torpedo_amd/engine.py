@@ -60,6 +60,20 @@ def gaussian_point(position, opacity, quaternion=(0.0, 0.0, 0.0, 1.0), scale=(1.
     return rec
 
 
+def from_model(ply_file) -> np.ndarray:
+    """GaussianPoint::fromModel (GaussianGeometry.cpp:59-127): a 3DGS PLY file as (N, 60) records
+    (tpd_load_ply: sigmoid opacity, normalised xyzw quaternion, exp scales, SH in file order)."""
+    recs = ctypes.POINTER(ctypes.c_float)()
+    n = ctypes.c_uint32()
+    check(lib().tpd_load_ply(str(ply_file).encode(), ctypes.byref(recs), ctypes.byref(n)))
+    try:
+        count = int(n.value)
+        out = np.ctypeslib.as_array(recs, shape=(max(count, 1) * GAUSSIAN_FLOATS,)).copy()[: count * GAUSSIAN_FLOATS]
+    finally:
+        lib().tpd_free_host(recs)
+    return out.reshape(count, GAUSSIAN_FLOATS)
+
+
 class Entity(int):
     """Handle returned by Scene.add (an entt::entity in the reference)."""
 
