@@ -270,7 +270,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
-    ap.add_argument("--only", choices=["multiview", "sort-load", "fwd-bwd", "multiscene"], default=None,
+    ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene"], default=None,
                     help="run one leg only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -295,6 +295,13 @@ def main():
         print(json.dumps({"workload": "clusters-1M-1920x1080 log-scale mean -3", "pairs": P2, "stage_ms": st2,
                           "onesweep_avg_launch_us": round(t_launch * 1e6, 2), "onesweep_gbs": round(ach, 1),
                           "copy_256MiB_gbs": round(copy_gbs, 1)}), flush=True)
+        eng.close()
+        return
+    if args.only == "plain":
+        g = scenes.cluster_scene(1_000_000, seed=2)
+        dt, P, st = time_plain(eng, g, eye, W, H, args.steps, args.warmup, tune=tune)
+        print(json.dumps({"workload": "clusters-1M-1920x1080", "ms_per_frame": round(1000.0 * dt / args.steps, 4),
+                          "pairs": P, "stage_ms": st}), flush=True)
         eng.close()
         return
     if args.only == "fwd-bwd":

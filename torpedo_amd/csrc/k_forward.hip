@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
     for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
         a.ranges[t] = make_uint2(0u, 0u);
     __syncthreads();
-    const uint32_t part = s_part;
+    const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);  // block-uniform: keep it scalar
     if (part >= gridDim.x) return;  // tickets are reset every frame; never true in a consistent state
     const uint32_t i = part * kPreBlock + tid;
 

@@ -70,14 +70,14 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
             const bool valid = i < n;
-            const uint64_t vmask = __ballot(valid);
+            const uint64_t vmask = ballot(valid);
 #pragma unroll
             for (uint32_t p = 0; p < kMaxPasses; ++p) {
                 if (p >= passes) break;
                 const uint32_t d = digit_of(k[u], p * kRadixBits);
                 const uint32_t prev = __shfl_up(d, 1, 64);
                 const bool head = valid && (lane == 0 || prev != d);
-                const uint64_t heads = __ballot(head) | ~vmask;  // invalid lanes end runs
+                const uint64_t heads = ballot(head) | ~vmask;  // invalid lanes end runs
                 if (head) {
                     const uint64_t nxt = heads & above;
                     const uint32_t end = nxt ? (uint32_t)__builtin_ctzll(nxt) : 64u;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     if (tid == 0) s_part = atomicAdd(a.ticket, 1u);
     for (uint32_t x = tid; x < kWaves * kRadix; x += BLOCK) (&s_whist[0][0])[x] = 0;
     __syncthreads();
-    const uint32_t part = s_part;
+    const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);  // block-uniform: keep it scalar
     if (part >= nparts) return;
     if (a.desc_next && tid < kRadix) a.desc_next[part * kRadix + tid] = 0;
 
@@ -155,15 +155,24 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const uint32_t wbase = base + wave * (ITEMS * kWave) + lane;
     uint64_t k[ITEMS];
     uint32_t v[ITEMS];
+    // Every partition but the last is full: its loads and stores need no bounds checks.
+    const bool full = base + kTileKeys <= n;
+    if (full) {
 #pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = wbase + j * kWave;
-        k[j] = idx < n ? a.keys_in[idx] : ~0ull;
-    }
+        for (uint32_t j = 0; j < ITEMS; ++j) k[j] = a.keys_in[wbase + j * kWave];
 #pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = wbase + j * kWave;
-        v[j] = idx < n ? a.vals_in[idx] : 0u;
+        for (uint32_t j = 0; j < ITEMS; ++j) v[j] = a.vals_in[wbase + j * kWave];
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < ITEMS; ++j) {
+            const uint32_t idx = wbase + j * kWave;
+            k[j] = idx < n ? a.keys_in[idx] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < ITEMS; ++j) {
+            const uint32_t idx = wbase + j * kWave;
+            v[j] = idx < n ? a.vals_in[idx] : 0u;
+        }
     }
 
     // Rank within the wave: peers with the same digit found with 8 ballots; the lowest lane of
@@ -179,7 +188,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
         for (uint32_t b = 0; b < kRadixBits; ++b) {
             const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
+            const uint64_t bal = ballot(bit);
             peers &= bit ? bal : ~bal;
         }
         const uint32_t below = (uint32_t)__popcll(peers & lt);
@@ -264,7 +273,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll 4
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (i < nvalid) a.keys_out[s_adj[s_dig[i]] + i] = s_keys[i];
+        if (full || i < nvalid) a.keys_out[s_adj[s_dig[i]] + i] = s_keys[i];
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
@@ -274,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll 4
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (i < nvalid) a.vals_out[s_adj[s_dig[i]] + i] = s_vals[i];
+        if (full || i < nvalid) a.vals_out[s_adj[s_dig[i]] + i] = s_vals[i];
     }
 }
 

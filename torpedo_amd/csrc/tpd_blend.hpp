@@ -87,7 +87,7 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
     const uint64_t lt = lanemask_lt();
     uint64_t bal[kBlendWaves];
 #pragma unroll
-    for (uint32_t q = 0; q < kBlendWaves; ++q) bal[q] = __ballot((mask >> q) & 1u);
+    for (uint32_t q = 0; q < kBlendWaves; ++q) bal[q] = ballot((mask >> q) & 1u);
     if (lane < kBlendWaves) {
         uint64_t b = bal[0];
 #pragma unroll
@@ -135,7 +135,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(done) == (int)kTilePixels) break;
         const uint32_t cnt = blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb);
-        if (__ballot(!done) == 0) continue;
+        if (ballot(!done) == 0) continue;
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t j = s.list[wave][i];
             const float4 A = s.a[j];
@@ -144,7 +144,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = !done && !(power > 0.0f) && !(power < B.z);
-            if (__ballot(act) == 0) continue;
+            if (ballot(act) == 0) continue;
             float alpha = B.y * soft_expf(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const bool ok = act && !(alpha < 1.0f / 255.0f);
@@ -157,7 +157,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
             cb = upd ? cb + (C.z * alpha) * T : cb;
             T = upd ? tn : T;
             done = done || stop;
-            if (__ballot(stop) != 0 && __ballot(!done) == 0) break;
+            if (ballot(stop) != 0 && ballot(!done) == 0) break;
         }
     }
     if (COUNT) {
@@ -225,7 +225,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(done) == (int)kTilePixels) break;
         const uint32_t cnt = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
-        if (__ballot(!done) == 0) continue;
+        if (ballot(!done) == 0) continue;
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t j = s.list[wave][i];
             const float4 A = s.a[j];
@@ -233,7 +233,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = !done && !(power > 0.0f) && !(power < B.z);
-            if (__ballot(act) == 0) continue;
+            if (ballot(act) == 0) continue;
             float alpha = B.y * soft_expf(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const bool ok = act && !(alpha < 1.0f / 255.0f);
@@ -243,7 +243,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             T = upd ? tn : T;
             last = upd ? (int32_t)(base + j) : last;
             done = done || stop;
-            if (__ballot(stop) != 0 && __ballot(!done) == 0) break;
+            if (ballot(stop) != 0 && ballot(!done) == 0) break;
         }
     }
     if (last >= 0) atomicMax(&L.last, last);
@@ -276,13 +276,13 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             const uint32_t j = s.list[wave][i];
             const int32_t kk = (int32_t)(base + j);
             const bool mine = kk <= last;
-            if (__ballot(mine) == 0) continue;
+            if (ballot(mine) == 0) continue;
             const float4 A = s.a[j];
             const float4 B = s.b[j];
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = mine && !(power > 0.0f) && !(power < B.z);
-            if (__ballot(act) == 0) continue;
+            if (ballot(act) == 0) continue;
             const float G = soft_expf(power);
             const float raw = B.y * G;
             const float alpha = raw < 0.99f ? raw : 0.99f;
@@ -313,7 +313,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                     v[4] = dL_dpower * (-0.5f * dy * dy);
                 }
             }
-            if (__ballot(upd) != 0) {
+            if (ballot(upd) != 0) {
 #pragma unroll
                 for (uint32_t q = 0; q < kG; ++q) {
                     const float sum = wave_sum_to_lane63(v[q]);

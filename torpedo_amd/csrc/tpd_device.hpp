@@ -39,10 +39,11 @@ __device__ __forceinline__ uint32_t as_uint(float f) { return __builtin_bit_cast
 // from the exponent bits.  Below -86 (and for NaN) it returns 0.  Identical bits on CPU and GPU
 // because it uses only correctly rounded operations.
 __device__ __forceinline__ float soft_expf(float x) {
-    if (!(x >= -86.0f)) return 0.0f;
-    if (x > 88.0f) return __builtin_inff();
-    const float n = __builtin_rintf(x * 1.44269504088896341f);
-    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    // The polynomial runs on an argument clamped to [-87, 89] (which changes nothing inside
+    // [-86, 88]) and the out-of-range results are selected afterwards, so there is no branch.
+    const float xc = __builtin_fminf(__builtin_fmaxf(x, -87.0f), 89.0f);
+    const float n = __builtin_rintf(xc * 1.44269504088896341f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, xc);
     r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
     float p = 1.0f / 5040.0f;
     p = __builtin_fmaf(p, r, 1.0f / 720.0f);
@@ -53,7 +54,9 @@ __device__ __forceinline__ float soft_expf(float x) {
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
     const int32_t e = (int32_t)n + 127;
-    return p * as_float((uint32_t)e << 23);
+    float y = p * as_float((uint32_t)e << 23);
+    y = (x >= -86.0f) ? y : 0.0f;  // also NaN -> 0
+    return (x > 88.0f) ? __builtin_inff() : y;
 }
 
 // Float -> UNORM8 for the RGBA8 target (blend.slang:104): clamp to [0, 1], scale by 255, round to
@@ -286,6 +289,9 @@ __device__ __forceinline__ bool splat_touches_rect(float gx, float gy, float a, 
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Wave-wide vote: bit l set when lane l's predicate holds (a scalar AND with exec).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const uint32_t l = __lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
@@ -388,8 +394,8 @@ __device__ __forceinline__ uint64_t scan_lookback(uint64_t* desc, uint32_t part)
         const int64_t idx = j - (int64_t)l;
         const uint64_t d = idx >= 0 ? load_relaxed(&desc[idx]) : kDescP;
         const uint32_t st = (uint32_t)(d >> 62);
-        const uint64_t mP = __ballot(st == 2);
-        const uint64_t mX = __ballot(st == 0);
+        const uint64_t mP = ballot(st == 2);
+        const uint64_t mX = ballot(st == 0);
         const uint32_t first_p = mP ? (uint32_t)__builtin_ctzll(mP) : 64u;
         const uint64_t upto = first_p >= 63 ? ~0ull : ((2ull << first_p) - 1ull);
         if (mX & upto) {
