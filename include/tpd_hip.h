@@ -148,7 +148,7 @@ int tpd_sort_pairs(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d
  * (results are identical for every shape).  sort_ms[0..capacity) receives the median time of a
  * full sort per shape; *n_shapes the number of shapes; *best the chosen one.  Re-renders the frame. */
 int tpd_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
-/* Selects a radix-pass block shape by index (0 = default). */
+/* Selects a radix-pass block shape by index (0 = default) for the plain and the batched frame. */
 int tpd_set_sort_shape(tpd_ctx* ctx, uint32_t shape);
 
 /* ----- batched rendering -------------------------------------------------------------------------

@@ -69,7 +69,18 @@ def test_sort_empty_and_invalid(eng):
         gpu_sort(eng, np.zeros(4, np.uint64), np.zeros(4, np.uint32), 0)
 
 
-@pytest.mark.parametrize("shape", range(6))
+NUM_SHAPES = 8  # 8-bit digits x 6 block shapes, 9-bit digits x 2
+
+
+def test_shape_count(eng):
+    from torpedo_amd._capi import TpdError
+    eng.setSortShape(NUM_SHAPES - 1)
+    with pytest.raises(TpdError):
+        eng.setSortShape(NUM_SHAPES)
+    eng.setSortShape(0)
+
+
+@pytest.mark.parametrize("shape", range(NUM_SHAPES))
 def test_every_block_shape(eng, shape):
     print(f"[sort] shape={shape}", flush=True)
     eng.setSortShape(shape)
@@ -84,5 +95,13 @@ def test_every_block_shape(eng, shape):
             k, v = gpu_sort(eng, keys, vals, 44)
             o = np.argsort(keys, kind="stable")
             assert np.array_equal(k, keys[o]) and np.array_equal(v, vals[o]), (shape, n)
+        # full 64-bit keys: the most passes any digit width needs
+        keys = rng.integers(0, 2 ** 63, 100_003, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 100_003,
+                                                                                                   dtype=np.uint64)
+        keys[::7] = keys[3]
+        vals = np.arange(keys.size, dtype=np.uint32)
+        k, v = gpu_sort(eng, keys, vals, 64)
+        o = np.argsort(keys, kind="stable")
+        assert np.array_equal(k, keys[o]) and np.array_equal(v, vals[o]), (shape, 64)
     finally:
         eng.setSortShape(0)

@@ -127,15 +127,26 @@ void batch_release(tpd_ctx* c) {
 }
 
 static BatchState* state(tpd_ctx* c) {
-    if (!c->batch) c->batch = new BatchState();
+    if (!c->batch) {
+        c->batch = new BatchState();
+        c->batch->ws.shape = c->ws.shape;  // the context's radix block shape until tuned separately
+    }
     return c->batch;
 }
+
 
 // Waits for a launched frame without validating it (before host-side table or buffer changes).
 static int wait_pending(BatchState* b) {
     if (!b->pending) return TPD_OK;
     hipError_t e = hipStreamSynchronize(b->pending_stream);
     return e == hipSuccess ? TPD_OK : hip_fail(e, "batch frame");
+}
+
+void batch_set_sort_shape(tpd_ctx* c, uint32_t shape) {
+    if (!c->batch) return;
+    (void)wait_pending(c->batch);
+    c->batch->ws.shape = shape;
+    destroy_graphs(c->batch);
 }
 
 // ------------------------------------------------------------------ layout
@@ -279,15 +290,15 @@ static int ensure_pairs(BatchState* b, size_t pairs) {
     const size_t cap = std::min({b->cap_keys, b->cap_vals, b->cap_keys_alt, b->cap_vals_alt, (size_t)kMaxSortKeys});
     const uint32_t parts = sort_parts((uint32_t)cap);
     if (!b->ws.partials) {
-        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess ||
-            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess ||
-            (e = hipMalloc(&b->ws.offsets, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess ||
+        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMalloc(&b->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
             (e = hipMalloc(&b->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
             return hip_fail(e, "tpd_batch: alloc sort workspace");
     }
     if (b->ws.parts_cap < parts) {
         release(b->ws.desc);
-        if ((e = hipMalloc(&b->ws.desc, sizeof(uint32_t) * 2 * (size_t)parts * kRadix)) != hipSuccess)
+        if ((e = hipMalloc(&b->ws.desc, sizeof(uint32_t) * 2 * (size_t)parts * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "tpd_batch: alloc sort workspace");
         b->ws.parts_cap = parts;
     }
@@ -362,7 +373,7 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     launch_batch_preprocess(a, s);
     if (events) hipEventRecord(b->ev[2], s);
     const uint32_t key_bits = batch_key_bits(b);
-    b->passes = sort_passes(key_bits);
+    b->passes = sort_shape_passes(b->ws.shape, key_bits);
     const bool in_alt = launch_sort_pairs(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
                                           &b->d_counters[kCtrPairs], b->cap, key_bits, s, events ? b->ev[3] : nullptr,
                                           31);

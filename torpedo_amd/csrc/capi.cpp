@@ -90,10 +90,10 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs) {
     const uint32_t parts = sort_parts((uint32_t)std::min<size_t>(pairs, kMaxSortKeys));
     hipError_t e;
     if (!c->ws.partials) {
-        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess ||
-            (e = hipMemset(c->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess)
+        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMemset(c->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort counts");
-        if ((e = hipMalloc(&c->ws.offsets, sizeof(uint32_t) * kMaxPasses * kRadix)) != hipSuccess)
+        if ((e = hipMalloc(&c->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort offsets");
         if ((e = hipMalloc(&c->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
             return hip_fail(e, "alloc sort tickets");
@@ -101,7 +101,7 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs) {
     if (c->ws.parts_cap < parts) {
         release(c->ws.desc);
         const uint32_t want = parts + parts / 4 + 1;
-        if ((e = hipMalloc(&c->ws.desc, sizeof(uint32_t) * 2 * (size_t)want * kRadix)) != hipSuccess)
+        if ((e = hipMalloc(&c->ws.desc, sizeof(uint32_t) * 2 * (size_t)want * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort descriptors");
         c->ws.parts_cap = want;
     }

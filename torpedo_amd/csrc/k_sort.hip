@@ -1,10 +1,11 @@
-// Stable LSD radix sort of (64-bit key, 32-bit value) pairs, 8-bit digits, one pass per digit.
+// Stable LSD radix sort of (64-bit key, 32-bit value) pairs, one pass per digit of BITS bits
+// (8 or 9; the autotuner picks the shape, digit width included).
 //
 // Replaces the reference's 2-bit "4-way" sorter (radix-shuffle/-prefixA/-prefixB/-mapping.slang,
 // 4 dispatches per 2 bits, GaussianEngine.cpp:819-841) with:
 //   k_sort_hist + k_sort_offsets  one read of the keys computes the digit histograms of every pass
 //                                 and their global exclusive offsets;
-//   k_onesweep (once per pass)    each 256-thread block takes the next 4096-key partition from an
+//   k_onesweep (once per pass)    each block takes the next partition of BLOCK x ITEMS keys from an
 //                                 atomic ticket, ranks its keys by digit with wave64 ballot
 //                                 matching (stable: element order inside a wave, then waves in
 //                                 order), publishes its per-digit counts with a decoupled
@@ -13,14 +14,17 @@
 // The result equals a stable sort by key, i.e. the reference's order (tile, depth, index).
 #include "tpd_kernels.hpp"
 
+#include <type_traits>
+
 namespace tpd {
 
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 
+template <uint32_t BITS>
 __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
-    return (uint32_t)(k >> shift) & (kRadix - 1u);
+    return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
 }
 
 // Keys are (high word << 32) | low word.  When the low word is known to use only lo_bits bits
@@ -37,21 +41,22 @@ __device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits) {
 // so each wave first collapses runs of equal digits (run heads found with one shuffle and one
 // ballot) and only run heads add their run length to the block's LDS histogram.  Each thread
 // keeps kHistUnroll loads in flight; blocks add their non-zero bins to the global counts
-// (kMaxPasses x kRadix, zero on entry; k_sort_offsets re-zeroes them after use).
+// (kMaxPasses x 2^BITS, zero on entry; k_sort_offsets re-zeroes them after use).
 constexpr uint32_t kHistUnroll = 8;
 
+template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
                                                    uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t tile_keys,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
                                                    uint32_t* __restrict__ tickets) {
-    __shared__ uint32_t h[kMaxPasses][kRadix];
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t h[kMaxPasses][R];
     const uint32_t n = count_ptr ? min(*count_ptr, n_imm) : n_imm;  // device count, capped by the capacity
     const uint32_t tid = threadIdx.x, lane = __lane_id();
-#pragma unroll
-    for (uint32_t p = 0; p < kMaxPasses; ++p) h[p][tid] = 0;
+    for (uint32_t x = tid; x < kMaxPasses * R; x += 256) (&h[0][0])[x] = 0;
     // The first onesweep pass needs region 0 of the look-back descriptors and all tickets at zero.
     const uint32_t nparts = (n + tile_keys - 1) / tile_keys;
-    for (uint32_t x = blockIdx.x * 256 + tid; x < nparts * kRadix; x += gridDim.x * 256) desc0[x] = 0;
+    for (uint32_t x = blockIdx.x * 256 + tid; x < nparts * R; x += gridDim.x * 256) desc0[x] = 0;
     if (blockIdx.x == 0 && tid < kMaxPasses) tickets[tid] = 0;
     __syncthreads();
 
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
 #pragma unroll
             for (uint32_t p = 0; p < kMaxPasses; ++p) {
                 if (p >= passes) break;
-                const uint32_t d = digit_of(k[u], p * kRadixBits);
+                const uint32_t d = digit_of<BITS>(k[u], p * BITS);
                 const uint32_t prev = __shfl_up(d, 1, 64);
                 const bool head = valid && (lane == 0 || prev != d);
                 const uint64_t heads = ballot(head) | ~vmask;  // invalid lanes end runs
@@ -87,25 +92,39 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
         }
     }
     __syncthreads();
-    for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t v = h[p][tid];
-        if (v) atomicAdd(&counts[p * kRadix + tid], v);
-    }
+    for (uint32_t p = 0; p < passes; ++p)
+        for (uint32_t d = tid; d < R; d += 256) {
+            const uint32_t v = h[p][d];
+            if (v) atomicAdd(&counts[p * R + d], v);
+        }
 }
 
-// One block per pass: exclusive scan of the digit counts, which are then re-zeroed.
+// One block per pass: exclusive scan of the 2^BITS digit counts (each thread takes a contiguous
+// run of digits), which are then re-zeroed.
+template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets) {
+    constexpr uint32_t R = 1u << BITS;
+    constexpr uint32_t DPT = R / 256;
     __shared__ uint32_t s_w[4];
-    const uint32_t p = blockIdx.x, d = threadIdx.x;
-    const uint32_t s = counts[p * kRadix + d];
-    counts[p * kRadix + d] = 0;
-    const uint32_t incl = wave_incl_scan(s);
-    const uint32_t w = d / kWave;
+    const uint32_t p = blockIdx.x, t = threadIdx.x;
+    uint32_t c[DPT], local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < DPT; ++i) {
+        c[i] = counts[p * R + t * DPT + i];
+        counts[p * R + t * DPT + i] = 0;
+        local += c[i];
+    }
+    const uint32_t incl = wave_incl_scan(local);
+    const uint32_t w = t / kWave;
     if (__lane_id() == kWave - 1) s_w[w] = incl;
     __syncthreads();
-    uint32_t off = 0;
+    uint32_t off = incl - local;
     for (uint32_t q = 0; q < w; ++q) off += s_w[q];
-    offsets[p * kRadix + d] = off + incl - s;
+#pragma unroll
+    for (uint32_t i = 0; i < DPT; ++i) {
+        offsets[p * R + t * DPT + i] = off;
+        off += c[i];
+    }
 }
 
 // ------------------------------------------------------------------ one pass
@@ -119,44 +138,97 @@ struct OnesweepArgs {
     uint32_t n_imm;
     uint32_t shift;
     uint32_t lo_bits;          // bits of the key's low word (32, or 31 for positive depths)
-    const uint32_t* offsets;   // kRadix global exclusive offsets of this pass's digits
-    uint32_t* desc;            // this pass's look-back region (parts x kRadix), zero on entry
+    const uint32_t* offsets;   // 2^BITS global exclusive offsets of this pass's digits
+    uint32_t* desc;            // this pass's look-back region (parts x 2^BITS), zero on entry
     uint32_t* desc_next;       // the other region, cleared here for the next pass
     uint32_t* ticket;
 };
 
-// BLOCK threads (256 or 512) x ITEMS keys per partition.  Threads 0..255 own one digit each for
-// the per-digit bookkeeping and the look-back.
-template <uint32_t BLOCK, uint32_t ITEMS>
+// LDS of one partition of BLOCK x ITEMS keys with 2^BITS digits.
+template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS>
+struct OnesweepLds {
+    static constexpr uint32_t R = 1u << BITS;
+    static constexpr uint32_t kWaves = BLOCK / kWave;
+    static constexpr uint32_t kTileKeys = BLOCK * ITEMS;
+    using Digit = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
+    uint16_t whist[kWaves][R];  // per wave digit counters (then wave offsets), zero on entry
+    uint32_t bstart[R];
+    uint32_t adj[R];
+    uint32_t wtot[8];
+    uint32_t part;
+    uint64_t keys[kTileKeys];   // the partition reordered by digit (then its values)
+    Digit dig[kTileKeys];
+};
+
+// Decoupled look-back of one digit: kLookback predecessor descriptors are loaded at once (before
+// partition 0 a virtual inclusive prefix of 0), consumed in order up to the first inclusive
+// prefix; an unpublished one is re-read after a short sleep.  Returns the exclusive prefix.
+template <uint32_t R>
+__device__ __forceinline__ uint32_t onesweep_lookback(const uint32_t* desc, uint32_t part, uint32_t d) {
+    constexpr int32_t kLookback = 8;
+    uint32_t excl = 0;
+    int32_t jj = (int32_t)part - 1;
+    while (true) {
+        uint32_t dsc[kLookback];
+#pragma unroll
+        for (int32_t u = 0; u < kLookback; ++u)
+            dsc[u] = jj - u >= 0 ? load_relaxed(&desc[(uint32_t)(jj - u) * R + d]) : kFlagP;
+        uint32_t add = 0;
+        int32_t used = 0;
+        bool found = false, blocked = false;
+#pragma unroll
+        for (int32_t u = 0; u < kLookback; ++u) {
+            if (!found && !blocked) {
+                const uint32_t f = dsc[u] & ~kValMask;
+                if (f == 0) {
+                    blocked = true;
+                } else {
+                    add += dsc[u] & kValMask;
+                    used = u + 1;
+                    found = f == kFlagP;
+                }
+            }
+        }
+        excl += add;
+        jj -= used;
+        if (found) return excl;
+        if (blocked) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// The digits are owned by DTH threads, each taking DPT consecutive digits for the per-digit
+// bookkeeping and the look-back.
+template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
-    constexpr uint32_t kWaves = BLOCK / kWave;
-    constexpr uint32_t kTileKeys = BLOCK * ITEMS;
-    static_assert(BLOCK == 256 || BLOCK == 512, "one or two threads per digit");
-    __shared__ uint32_t s_part;
-    __shared__ uint32_t s_whist[kWaves][kRadix];
-    __shared__ uint32_t s_bstart[kRadix];
-    __shared__ uint32_t s_adj[kRadix];
-    __shared__ uint32_t s_wtot[4];
-    __shared__ uint64_t s_keys[kTileKeys];
-    __shared__ uint8_t s_dig[kTileKeys];
+    using Lds = OnesweepLds<BITS, BLOCK, ITEMS>;
+    constexpr uint32_t R = Lds::R;
+    constexpr uint32_t kWaves = Lds::kWaves;
+    constexpr uint32_t kTileKeys = Lds::kTileKeys;
+    constexpr uint32_t DTH = R < BLOCK ? R : BLOCK;  // threads owning digits
+    constexpr uint32_t DPT = R / DTH;                // digits per owning thread
+    static_assert(R >= 256 && (R % BLOCK == 0 || BLOCK % R == 0), "digits and threads must tile");
+    static_assert(DTH / kWave <= 8, "at most 8 digit-owning waves");
+    __shared__ Lds L;
 
     const uint32_t n = a.count_ptr ? min(*a.count_ptr, a.n_imm) : a.n_imm;
     const uint32_t nparts = (n + kTileKeys - 1) / kTileKeys;
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
-    if (tid == 0) s_part = atomicAdd(a.ticket, 1u);
-    for (uint32_t x = tid; x < kWaves * kRadix; x += BLOCK) (&s_whist[0][0])[x] = 0;
+    if (tid == 0) L.part = atomicAdd(a.ticket, 1u);
+    for (uint32_t x = tid; x < kWaves * R / 2; x += BLOCK) reinterpret_cast<uint32_t*>(&L.whist[0][0])[x] = 0;
     __syncthreads();
-    const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);  // block-uniform: keep it scalar
+    const uint32_t part = __builtin_amdgcn_readfirstlane(L.part);  // block-uniform: keep it scalar
     if (part >= nparts) return;
-    if (a.desc_next && tid < kRadix) a.desc_next[part * kRadix + tid] = 0;
+    if (a.desc_next)
+        for (uint32_t d = tid; d < R; d += BLOCK) a.desc_next[part * R + d] = 0;
 
-    // Load ITEMS keys per thread, wave-striped: element e = wave*(ITEMS*64) + j*64 + lane.
+    // Load ITEMS keys and values per thread, wave-striped: element e = wave*(ITEMS*64) + j*64 +
+    // lane; past the end keys are all ones.  Every partition but the last is full: its loads and
+    // stores need no bounds checks.
     const uint32_t base = part * kTileKeys;
     const uint32_t wbase = base + wave * (ITEMS * kWave) + lane;
+    const bool full = base + kTileKeys <= n;
     uint64_t k[ITEMS];
     uint32_t v[ITEMS];
-    // Every partition but the last is full: its loads and stores need no bounds checks.
-    const bool full = base + kTileKeys <= n;
     if (full) {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) k[j] = a.keys_in[wbase + j * kWave];
@@ -175,47 +247,61 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
         }
     }
 
-    // Rank within the wave: peers with the same digit found with 8 ballots; the lowest lane of
+    // Rank within the wave: peers with the same digit found with BITS ballots; the lowest lane of
     // each group advances the wave's running digit counter.
     const uint64_t lt = lanemask_lt();
     uint32_t rank[ITEMS], dig[ITEMS];
 #pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) dig[j] = digit_of(compact_key(k[j], a.lo_bits), a.shift);
+    for (uint32_t j = 0; j < ITEMS; ++j) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits), a.shift);
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = dig[j];
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (uint32_t b = 0; b < kRadixBits; ++b) {
+        for (uint32_t b = 0; b < BITS; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = ballot(bit);
             peers &= bit ? bal : ~bal;
         }
         const uint32_t below = (uint32_t)__popcll(peers & lt);
-        const uint32_t pre = s_whist[wave][d];
-        if (below == 0) s_whist[wave][d] = pre + (uint32_t)__popcll(peers);
+        const uint32_t pre = L.whist[wave][d];
+        if (below == 0) L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
         rank[j] = pre + below;
     }
     __syncthreads();
 
-    // Per digit (thread = digit): wave exclusive offsets and the partition's digit count.
-    uint32_t tot = 0, bstart = 0;
-    if (tid < kRadix) {
+    // Per digit: wave exclusive offsets and the partition's digit counts (published); then the
+    // digits' block-local starts by an exclusive scan over the digit-owning threads.
+    uint32_t tot[DPT];
+    uint32_t tstart = 0;
+    if (tid < DTH) {
+        uint32_t local = 0;
 #pragma unroll
-        for (uint32_t w = 0; w < kWaves; ++w) {
-            const uint32_t c = s_whist[w][tid];
-            s_whist[w][tid] = tot;
-            tot += c;
+        for (uint32_t i = 0; i < DPT; ++i) {
+            const uint32_t d = tid * DPT + i;
+            uint32_t acc = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                const uint32_t c = L.whist[w][d];
+                L.whist[w][d] = (uint16_t)acc;
+                acc += c;
+            }
+            tot[i] = acc;
+            local += acc;
+            store_relaxed(&a.desc[part * R + d], (part == 0 ? kFlagP : kFlagA) | acc);
         }
-        store_relaxed(&a.desc[part * kRadix + tid], (part == 0 ? kFlagP : kFlagA) | tot);
-        const uint32_t incl = wave_incl_scan(tot);
-        if (lane == kWave - 1) s_wtot[wave] = incl;
-        bstart = incl - tot;
+        const uint32_t incl = wave_incl_scan(local);
+        if (lane == kWave - 1) L.wtot[wave] = incl;
+        tstart = incl - local;
     }
     __syncthreads();
-    if (tid < kRadix) {
-        for (uint32_t w = 0; w < wave; ++w) bstart += s_wtot[w];
-        s_bstart[tid] = bstart;
+    if (tid < DTH) {
+        for (uint32_t w = 0; w < wave; ++w) tstart += L.wtot[w];
+#pragma unroll
+        for (uint32_t i = 0; i < DPT; ++i) {
+            L.bstart[tid * DPT + i] = tstart;
+            tstart += tot[i];
+        }
     }
     __syncthreads();
 
@@ -223,49 +309,24 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = dig[j];
-        const uint32_t lp = s_bstart[d] + s_whist[wave][d] + rank[j];
+        const uint32_t lp = L.bstart[d] + L.whist[wave][d] + rank[j];
         rank[j] = lp;
-        s_keys[lp] = k[j];
-        s_dig[lp] = (uint8_t)d;
+        L.keys[lp] = k[j];
+        L.dig[lp] = (typename Lds::Digit)d;
     }
 
-    // Decoupled look-back for digit tid: kLookback predecessor descriptors are loaded at once
-    // (before partition 0 a virtual inclusive prefix of 0), consumed in order up to the first
-    // inclusive prefix; an unpublished one is re-read after a short sleep.
-    if (tid < kRadix) {
-        constexpr int32_t kLookback = 8;
-        uint32_t excl = 0;
-        if (part > 0) {
-            int32_t jj = (int32_t)part - 1;
-            while (true) {
-                uint32_t dsc[kLookback];
+    // Look-back per digit, then the global position adjustment of each digit run.
+    if (tid < DTH) {
 #pragma unroll
-                for (int32_t u = 0; u < kLookback; ++u)
-                    dsc[u] = jj - u >= 0 ? load_relaxed(&a.desc[(uint32_t)(jj - u) * kRadix + tid]) : kFlagP;
-                uint32_t add = 0;
-                int32_t used = 0;
-                bool found = false, blocked = false;
-#pragma unroll
-                for (int32_t u = 0; u < kLookback; ++u) {
-                    if (!found && !blocked) {
-                        const uint32_t f = dsc[u] & ~kValMask;
-                        if (f == 0) {
-                            blocked = true;
-                        } else {
-                            add += dsc[u] & kValMask;
-                            used = u + 1;
-                            found = f == kFlagP;
-                        }
-                    }
-                }
-                excl += add;
-                jj -= used;
-                if (found) break;
-                if (blocked) __builtin_amdgcn_s_sleep(1);
+        for (uint32_t i = 0; i < DPT; ++i) {
+            const uint32_t d = tid * DPT + i;
+            uint32_t excl = 0;
+            if (part > 0) {
+                excl = onesweep_lookback<R>(a.desc, part, d);
+                store_relaxed(&a.desc[part * R + d], kFlagP | (excl + tot[i]));
             }
-            store_relaxed(&a.desc[part * kRadix + tid], kFlagP | (excl + tot));
+            L.adj[d] = a.offsets[d] + excl - L.bstart[d];
         }
-        s_adj[tid] = a.offsets[tid] + excl - bstart;
     }
     __syncthreads();
 
@@ -273,47 +334,62 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll 4
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) a.keys_out[s_adj[s_dig[i]] + i] = s_keys[i];
+        if (full || i < nvalid) a.keys_out[L.adj[L.dig[i]] + i] = L.keys[i];
     }
     __syncthreads();
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) s_vals[rank[j]] = v[j];
     __syncthreads();
 #pragma unroll 4
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) a.vals_out[s_adj[s_dig[i]] + i] = s_vals[i];
+        if (full || i < nvalid) a.vals_out[L.adj[L.dig[i]] + i] = s_vals[i];
     }
 }
 
-// Block shapes the tuner chooses from: {threads, keys per thread}.
+// Shapes the tuner chooses from: {digit bits, threads, keys per thread}.
 struct SortShape {
-    uint32_t block, items;
+    uint32_t bits, block, items;
     void (*kernel)(const OnesweepArgs);
 };
 static const SortShape kShapes[] = {
-    {256, 16, k_onesweep<256, 16>}, {256, 8, k_onesweep<256, 8>},   {256, 12, k_onesweep<256, 12>},
-    {512, 8, k_onesweep<512, 8>},   {512, 12, k_onesweep<512, 12>}, {256, 20, k_onesweep<256, 20>},
+    {8, 256, 16, k_onesweep<8, 256, 16>}, {8, 256, 8, k_onesweep<8, 256, 8>},   {8, 256, 12, k_onesweep<8, 256, 12>},
+    {8, 512, 8, k_onesweep<8, 512, 8>},   {8, 512, 12, k_onesweep<8, 512, 12>}, {8, 256, 20, k_onesweep<8, 256, 20>},
+    {9, 512, 12, k_onesweep<9, 512, 12>}, {9, 512, 8, k_onesweep<9, 512, 8>},
 };
 constexpr uint32_t kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 uint32_t sort_num_shapes() { return kNumShapes; }
 uint32_t sort_shape_keys(uint32_t shape) { return kShapes[shape % kNumShapes].block * kShapes[shape % kNumShapes].items; }
+uint32_t sort_shape_bits(uint32_t shape) { return kShapes[shape % kNumShapes].bits; }
+uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits) {
+    const uint32_t b = sort_shape_bits(shape);
+    return (key_bits + b - 1) / b;
+}
+
+template <uint32_t BITS>
+static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const uint64_t* keys, const uint32_t* count_ptr,
+                        uint32_t n_max, uint32_t lo_bits, uint32_t tile, SortWorkspace& ws) {
+    hipLaunchKernelGGL(k_sort_hist<BITS>, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, tile,
+                       ws.partials, ws.desc, ws.tickets);
+    hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+}
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
                        hipEvent_t after_hist, uint32_t lo_bits) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
+    const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
-    const uint32_t passes = sort_passes(key_bits);
+    const uint32_t passes = (key_bits + sh.bits - 1) / sh.bits;
     const uint32_t parts = (n_max + tile - 1) / tile;
-    if (parts == 0 || passes == 0) return false;
+    if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
+
     uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
     hb = hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
-    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, tile, ws.partials,
-                       ws.desc, ws.tickets);
-    hipLaunchKernelGGL(k_sort_offsets, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
+    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
     const uint64_t* kin = keys;
     const uint32_t* vin = vals;
@@ -323,11 +399,11 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         OnesweepArgs o;
         o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
         o.count_ptr = count_ptr; o.n_imm = n_max;
-        o.shift = p * kRadixBits;
+        o.shift = p * sh.bits;
         o.lo_bits = lo_bits;
-        o.offsets = ws.offsets + p * kRadix;
-        o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * kRadix;
-        o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * kRadix;
+        o.offsets = ws.offsets + p * R;
+        o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
+        o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * R;
         o.ticket = ws.tickets + p;
         hipLaunchKernelGGL(sh.kernel, dim3(parts), dim3(sh.block), 0, s, o);
         const uint64_t* tk = kout; kout = const_cast<uint64_t*>(kin); kin = tk;

@@ -93,12 +93,14 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     }
 
     bool in_alt = false;
-    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 32;
-    c->last_passes = P > 0 ? sort_passes(key_bits) : 0;
+    // Emitted depths are positive floats (view z > 0), so bit 31 of every key's low word is clear
+    // and the sort may run on the compacted (tile << 31) | depth key: same order, one bit less.
+    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 31;
+    c->last_passes = P > 0 ? sort_shape_passes(c->ws.shape, key_bits) : 0;
     if (c->profile) hipEventRecord(c->ev[2], s);
     if (P > 0)
         in_alt = launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, P, key_bits, s,
-                                   c->profile ? c->ev[3] : nullptr);
+                                   c->profile ? c->ev[3] : nullptr, 31);
     else if (c->profile)
         hipEventRecord(c->ev[3], s);
     if (c->profile) hipEventRecord(c->ev[4], s);

@@ -85,6 +85,7 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs);
 int render_forward(tpd_ctx* c, hipStream_t s);
 void backward_release(tpd_ctx* c);
 void batch_release(tpd_ctx* c);
+void batch_set_sort_shape(tpd_ctx* c, uint32_t shape);
 
 // Grows a device allocation to hold at least `count` elements (contents are not preserved).
 template <typename T>

@@ -194,18 +194,18 @@ void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s);
 
 // ------------------------------------------------------------------ radix sort
 
-constexpr uint32_t kRadixBits = 8;
-constexpr uint32_t kRadix = 1u << kRadixBits;
+constexpr uint32_t kMaxRadixBits = 9;                      // digit widths 8 and 9 bits
+constexpr uint32_t kMaxRadix = 1u << kMaxRadixBits;
 constexpr uint32_t kSortMinTile = 2048;                    // smallest partition of any block shape
 constexpr uint32_t kMaxPasses = 8;
 constexpr uint32_t kHistBlocks = 512;                      // blocks of the digit-histogram kernel
 constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back values
 
 struct SortWorkspace {
-    uint32_t* desc = nullptr;      // 2 regions x parts_cap x kRadix look-back descriptors
+    uint32_t* desc = nullptr;      // 2 regions x parts_cap x kMaxRadix look-back descriptors
     uint32_t parts_cap = 0;
-    uint32_t* partials = nullptr;  // kMaxPasses x kRadix global digit counts (zero between sorts)
-    uint32_t* offsets = nullptr;   // kMaxPasses x kRadix global exclusive digit offsets
+    uint32_t* partials = nullptr;  // kMaxPasses x kMaxRadix global digit counts (zero between sorts)
+    uint32_t* offsets = nullptr;   // kMaxPasses x kMaxRadix global exclusive digit offsets
     uint32_t* tickets = nullptr;   // kMaxPasses partition tickets
     uint32_t shape = 0;            // onesweep block shape (index into the tuner's table)
 };
@@ -214,7 +214,8 @@ struct SortWorkspace {
 __host__ __device__ inline uint32_t sort_parts(uint32_t n) { return (n + kSortMinTile - 1) / kSortMinTile; }
 uint32_t sort_num_shapes();
 uint32_t sort_shape_keys(uint32_t shape);
-__host__ __device__ inline uint32_t sort_passes(uint32_t key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
+uint32_t sort_shape_bits(uint32_t shape);                      // digit width of a shape
+uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits);  // passes for key_bits bits
 
 // Stable LSD sort of (keys, vals) by the low key_bits bits of the keys.  n comes from *count_ptr
 // when count_ptr is non-null (device-resident count capped at n_max, grids sized by n_max), else

@@ -49,6 +49,7 @@ int tpd_set_sort_shape(tpd_ctx* c, uint32_t shape) {
     if (!c) return fail(TPD_ERR_INVALID, "null context");
     if (shape >= sort_num_shapes()) return fail(TPD_ERR_INVALID, "tpd_set_sort_shape: no such shape");
     c->ws.shape = shape;
+    batch_set_sort_shape(c, shape);
     return TPD_OK;
 }
 
@@ -62,7 +63,7 @@ int tpd_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shap
     hipStream_t s = c->stream;
     const uint32_t W = c->cam.width, H = c->cam.height;
     const uint32_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
-    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 32;
+    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 31;  // compacted keys, as render_forward
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(TPD_ERR_HIP, "tpd_autotune: hipEventCreate");
@@ -76,7 +77,7 @@ int tpd_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shap
             if ((rc = rerun_duplicate(c, s)) != TPD_OK) break;
             hipEventRecord(e0, s);
             launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, c->last_pairs,
-                              key_bits, s);
+                              key_bits, s, nullptr, 31);
             hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) {
                 rc = fail(TPD_ERR_HIP, "tpd_autotune: sort failed");
