@@ -145,7 +145,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = !done && !(power > 0.0f) && !(power < B.z);
             if (ballot(act) == 0) continue;
-            float alpha = B.y * soft_expf(power);
+            float alpha = B.y * soft_expf_blend(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const bool ok = act && !(alpha < 1.0f / 255.0f);
             const float tn = T * (1.0f - alpha);
@@ -234,7 +234,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = !done && !(power > 0.0f) && !(power < B.z);
             if (ballot(act) == 0) continue;
-            float alpha = B.y * soft_expf(power);
+            float alpha = B.y * soft_expf_blend(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const bool ok = act && !(alpha < 1.0f / 255.0f);
             const float tn = T * (1.0f - alpha);
@@ -283,7 +283,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const bool act = mine && !(power > 0.0f) && !(power < B.z);
             if (ballot(act) == 0) continue;
-            const float G = soft_expf(power);
+            const float G = soft_expf_blend(power);
             const float raw = B.y * G;
             const float alpha = raw < 0.99f ? raw : 0.99f;
             const bool upd = act && !(alpha < 1.0f / 255.0f);
@@ -314,10 +314,11 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                 }
             }
             if (ballot(upd) != 0) {
+                // row sums by DPP, then one LDS atomic per row of 16 lanes
 #pragma unroll
                 for (uint32_t q = 0; q < kG; ++q) {
-                    const float sum = wave_sum_to_lane63(v[q]);
-                    if (lane == 63 && sum != 0.0f) atomicAdd(&L.acc[j][q], sum);
+                    const float sum = row_sum16(v[q]);
+                    if ((lane & 15u) == 15u && sum != 0.0f) atomicAdd(&L.acc[j][q], sum);
                 }
             }
         }

@@ -59,6 +59,26 @@ __device__ __forceinline__ float soft_expf(float x) {
     return (x > 88.0f) ? __builtin_inff() : y;
 }
 
+// soft_expf for the blend, where every lane that uses the result has an argument in
+// [log(1/(255 o)) - 0.01, 0] (o >= 1/255) or NaN: the same bits as soft_expf there, without the
+// clamp and the overflow select that such arguments never reach (other lanes' results are unused).
+__device__ __forceinline__ float soft_expf_blend(float x) {
+    const float n = __builtin_rintf(x * 1.44269504088896341f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
+    float p = 1.0f / 5040.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const int32_t e = (int32_t)n + 127;
+    const float y = p * as_float((uint32_t)e << 23);
+    return (x >= -86.0f) ? y : 0.0f;  // NaN -> 0
+}
+
 // Float -> UNORM8 for the RGBA8 target (blend.slang:104): clamp to [0, 1], scale by 255, round to
 // nearest even.
 __device__ __forceinline__ uint8_t unorm8(float v) {
@@ -311,6 +331,20 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Sum over each row of 16 lanes with DPP moves only: quad_perm [1,0,3,2] and [2,3,0,1] -> quad
+// sums, row_half_mirror -> 8-lane sums, row_mirror -> every lane holds its row's sum.
+__device__ __forceinline__ float row_sum16(float v) {
+    int x = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    x = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    x = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
+    x = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));
     return v;
 }
 
