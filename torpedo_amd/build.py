@@ -38,6 +38,10 @@ def command(out: str = LIB) -> list[str]:
     return [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-x", "hip",
             "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # The kernels reduce across lanes themselves before every atomic to a wave-uniform
+            # address; the compiler's own lane-by-lane reduction loop around such atomics only adds
+            # instructions (one per active lane, per atomic).
+            "-Xarch_device", "-mllvm=-amdgpu-atomic-optimizer-strategy=None",
             "-Wno-unused-command-line-argument",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + ["-o", out]
 

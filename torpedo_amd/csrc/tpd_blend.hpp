@@ -22,7 +22,7 @@ constexpr uint32_t kSplatGrads = 9;  // dL/d(image x, y, conic a, b, c, r, g, b,
 
 struct BlendLds {
     float4 a[kTilePixels];  // x, y, conic a, conic b
-    float4 b[kTilePixels];  // conic c, opacity, skip threshold, -
+    float4 b[kTilePixels];  // conic c, skip threshold, opacity, -
     float4 c[kTilePixels];  // colour
     uint8_t list[kBlendWaves][kTilePixels];
     uint32_t cnt[kBlendWaves][kBlendWaves];  // [staging wave][quadrant]
@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
         const float4 p = xyzr[g];
         const float4 c = conop[g];
         s.a[tid] = make_float4(p.x, p.y, c.x, c.y);
-        s.b[tid] = make_float4(c.z, c.w, __logf(1.0f / (255.0f * c.w)) - 0.01f, 0.0f);
+        s.b[tid] = make_float4(c.z, __logf(1.0f / (255.0f * c.w)) - 0.01f, c.w, 0.0f);
         s.c[tid] = rgb[g];
         if (STORE_G) s_g[tid] = g;
         // Bounding box of the region Q <= q0 outside which splat_touches_rect always says no
@@ -129,35 +129,43 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     const bool inside = pxl < tv.width && pyl < tv.height;
     const float fx = (float)pxl, fy = (float)pyl;
 
+    // The per-pixel state machine runs on wave-uniform lane masks (scalar instructions) rather
+    // than per-lane booleans: live = pixels not yet terminated.
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    bool done = !inside;
+    uint64_t live = ballot(inside);
     uint32_t evals = 0;
+    const uint8_t* list = s.list[__builtin_amdgcn_readfirstlane(wave)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
-        if (__syncthreads_count(done) == (int)kTilePixels) break;
-        const uint32_t cnt = blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb);
-        if (ballot(!done) == 0) continue;
+        if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
+            blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb));
+        if (live == 0) continue;
         for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t j = s.list[wave][i];
+            const uint32_t j = list[i];
             const float4 A = s.a[j];
-            const float4 B = s.b[j];
-            if (COUNT) evals += done ? 0u : 1u;
+            const float2 B = *reinterpret_cast<const float2*>(&s.b[j]);  // conic c, skip threshold
+            if (COUNT) evals += lane_on(live) ? 1u : 0u;
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            const bool act = !done && !(power > 0.0f) && !(power < B.z);
-            if (ballot(act) == 0) continue;
-            float alpha = B.y * soft_expf_blend(power);
+            // power <= 0 equals !(power > 0) except for NaN, whose exp is 0 and whose alpha
+            // therefore fails the 1/255 test: either way the splat is skipped.  Active lanes have
+            // power in [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf's plain range.
+            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
+            if (act == 0) continue;
+            float alpha = s.b[j].z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
-            const bool ok = act && !(alpha < 1.0f / 255.0f);
             const float tn = T * (1.0f - alpha);
-            const bool stop = ok && tn < 0.0001f;
-            const bool upd = ok && !(tn < 0.0001f);
-            const float4 C = s.c[j];
-            cr = upd ? cr + (C.x * alpha) * T : cr;
-            cg = upd ? cg + (C.y * alpha) * T : cg;
-            cb = upd ? cb + (C.z * alpha) * T : cb;
-            T = upd ? tn : T;
-            done = done || stop;
-            if (ballot(stop) != 0 && ballot(!done) == 0) break;
+            const uint64_t ok = act & ballot(!(alpha < 1.0f / 255.0f));
+            const uint64_t low = ballot(tn < 0.0001f);
+            if (lane_on(ok & ~low)) {
+                const float4 C = s.c[j];
+                cr = cr + (C.x * alpha) * T;
+                cg = cg + (C.y * alpha) * T;
+                cb = cb + (C.z * alpha) * T;
+                T = tn;
+            }
+            live &= ~(ok & low);
+            if (live == 0) break;
         }
     }
     if (COUNT) {
@@ -220,30 +228,33 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
 
     // ---- pass 1: forward re-run (same decisions as blend_tile) -> T_final, last contributor
     float T = 1.0f;
-    bool done = !inside;
+    uint64_t live = ballot(inside);
     int32_t last = -1;
+    const uint8_t* list = s.list[__builtin_amdgcn_readfirstlane(wave)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
-        if (__syncthreads_count(done) == (int)kTilePixels) break;
-        const uint32_t cnt = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
-        if (ballot(!done) == 0) continue;
+        if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
+            blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb));
+        if (live == 0) continue;
         for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t j = s.list[wave][i];
+            const uint32_t j = list[i];
             const float4 A = s.a[j];
-            const float4 B = s.b[j];
+            const float2 B = *reinterpret_cast<const float2*>(&s.b[j]);
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            const bool act = !done && !(power > 0.0f) && !(power < B.z);
-            if (ballot(act) == 0) continue;
-            float alpha = B.y * soft_expf_blend(power);
+            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
+            if (act == 0) continue;
+            float alpha = s.b[j].z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
-            const bool ok = act && !(alpha < 1.0f / 255.0f);
             const float tn = T * (1.0f - alpha);
-            const bool stop = ok && tn < 0.0001f;
-            const bool upd = ok && !(tn < 0.0001f);
-            T = upd ? tn : T;
-            last = upd ? (int32_t)(base + j) : last;
-            done = done || stop;
-            if (ballot(stop) != 0 && ballot(!done) == 0) break;
+            const uint64_t ok = act & ballot(!(alpha < 1.0f / 255.0f));
+            const uint64_t low = ballot(tn < 0.0001f);
+            if (lane_on(ok & ~low)) {
+                T = tn;
+                last = (int32_t)(base + j);
+            }
+            live &= ~(ok & low);
+            if (live == 0) break;
         }
     }
     if (last >= 0) atomicMax(&L.last, last);
@@ -271,22 +282,24 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     for (int32_t chunk = (int32_t)first_chunk; chunk >= 0; --chunk) {
         const uint32_t base = range.x + (uint32_t)chunk * kTilePixels;
         __syncthreads();
-        const uint32_t cnt = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
+            blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb));
         for (int32_t i = (int32_t)cnt - 1; i >= 0; --i) {
-            const uint32_t j = s.list[wave][i];
+            const uint32_t j = list[i];
             const int32_t kk = (int32_t)(base + j);
-            const bool mine = kk <= last;
-            if (ballot(mine) == 0) continue;
+            const uint64_t mine = ballot(kk <= last);
+            if (mine == 0) continue;
             const float4 A = s.a[j];
             const float4 B = s.b[j];
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            const bool act = mine && !(power > 0.0f) && !(power < B.z);
-            if (ballot(act) == 0) continue;
-            const float G = soft_expf_blend(power);
-            const float raw = B.y * G;
+            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & mine;
+            if (act == 0) continue;
+            const float G = soft_expf_inrange(power);
+            const float raw = B.z * G;
             const float alpha = raw < 0.99f ? raw : 0.99f;
-            const bool upd = act && !(alpha < 1.0f / 255.0f);
+            const uint64_t updm = act & ballot(!(alpha < 1.0f / 255.0f));
+            const bool upd = lane_on(updm);
             float v[kG];
 #pragma unroll
             for (uint32_t q = 0; q < kG; ++q) v[q] = 0.0f;
@@ -313,7 +326,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                     v[4] = dL_dpower * (-0.5f * dy * dy);
                 }
             }
-            if (ballot(upd) != 0) {
+            if (updm != 0) {
                 // row sums by DPP, then one LDS atomic per row of 16 lanes
 #pragma unroll
                 for (uint32_t q = 0; q < kG; ++q) {

@@ -59,6 +59,24 @@ __device__ __forceinline__ float soft_expf(float x) {
     return (x > 88.0f) ? __builtin_inff() : y;
 }
 
+// The polynomial and scaling of soft_expf alone: the same bits as soft_expf for x in [-86, 88];
+// garbage (never a fault) outside.
+__device__ __forceinline__ float soft_expf_inrange(float x) {
+    const float n = __builtin_rintf(x * 1.44269504088896341f);
+    float r = __builtin_fmaf(n, -0.693145751953125f, x);
+    r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
+    float p = 1.0f / 5040.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const int32_t e = (int32_t)n + 127;
+    return p * as_float((uint32_t)e << 23);
+}
+
 // soft_expf for the blend, where every lane that uses the result has an argument in
 // [log(1/(255 o)) - 0.01, 0] (o >= 1/255) or NaN: the same bits as soft_expf there, without the
 // clamp and the overflow select that such arguments never reach (other lanes' results are unused).
@@ -311,6 +329,10 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Wave-wide vote: bit l set when lane l's predicate holds (a scalar AND with exec).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// The inverse: this lane's bit of a wave-uniform mask, as a predicate held in a scalar lane mask
+// (no vector instruction).
+__device__ __forceinline__ bool lane_on(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const uint32_t l = __lane_id();
