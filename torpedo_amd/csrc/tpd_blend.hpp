@@ -18,14 +18,22 @@ struct TileView {
 
 constexpr uint32_t kBlendWaves = kTilePixels / kWave;  // 4 waves, one 8x8 quadrant each
 constexpr uint32_t kQuad = 8;
+constexpr uint32_t kSub = 4;                              // 4x4-pixel sub-blocks ...
+constexpr uint32_t kSubBlocks = kTilePixels / (kSub * kSub);  // ... 16 per tile, one per 16-lane row
+constexpr uint32_t kSlotBytes = 16;                       // one float4 per staged splat and array
+constexpr uint32_t kSentinel = kTilePixels * kSlotBytes;  // byte offset of the slot no pixel takes
 constexpr uint32_t kSplatGrads = 9;  // dL/d(image x, y, conic a, b, c, r, g, b, opacity)
 
+// A staging round holds up to 256 splats of the tile's sorted list in slots 0..255; slot 256 is a
+// sentinel that no pixel can take (skip threshold +inf).  Sub-block lists hold the byte offsets
+// (slot * 16) of the splats that can reach the sub-block, in sorted order, padded with the
+// sentinel up to the longest list of the wave.
 struct BlendLds {
-    float4 a[kTilePixels];  // x, y, conic a, conic b
-    float4 b[kTilePixels];  // conic c, skip threshold, opacity, -
-    float4 c[kTilePixels];  // colour
-    uint8_t list[kBlendWaves][kTilePixels];
-    uint32_t cnt[kBlendWaves][kBlendWaves];  // [staging wave][quadrant]
+    float4 a[kTilePixels + 1];  // x, y, conic a, conic b
+    float4 b[kTilePixels + 1];  // conic c, skip threshold, opacity, -
+    float4 c[kTilePixels + 1];  // colour
+    uint16_t list[kSubBlocks][kTilePixels];
+    uint32_t cnt[kBlendWaves][kSubBlocks];  // [staging wave][sub-block]
 };
 
 struct BlendBwdLds {
@@ -35,23 +43,53 @@ struct BlendBwdLds {
     int32_t last;
 };
 
-// Pixel of this thread: wave w owns the 8x8 quadrant (w & 1, w >> 1) of the tile.
+__device__ __forceinline__ float4 slot4(const float4* arr, uint32_t off) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(arr) + off);
+}
+__device__ __forceinline__ float2 slot2(const float4* arr, uint32_t off) {
+    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(arr) + off);
+}
+
+// Sub-block q of the tile (0..15): quadrant q >> 2 (wave), 4x4 block q & 3 inside it.
+__device__ __forceinline__ void sub_block_origin(const TileView& tv, uint32_t q, uint32_t& x0, uint32_t& y0) {
+    const uint32_t w = q >> 2, g = q & 3u;
+    x0 = tv.tx * kTile + (w & 1u) * kQuad + (g & 1u) * kSub;
+    y0 = tv.ty * kTile + (w >> 1) * kQuad + (g >> 1) * kSub;
+}
+
+// Pixel of this thread: wave w owns the 8x8 quadrant (w & 1, w >> 1) of the tile, the 16-lane row
+// r of the wave its 4x4 sub-block (r & 1, r >> 1), lanes in row-major order inside it.
 __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uint32_t& py) {
     const uint32_t wave = threadIdx.x / kWave, lane = __lane_id();
-    px = tv.tx * kTile + (wave & 1u) * kQuad + (lane % kQuad);
-    py = tv.ty * kTile + (wave >> 1) * kQuad + (lane / kQuad);
+    uint32_t x0, y0;
+    sub_block_origin(tv, wave * 4u + (lane >> 4), x0, y0);
+    px = x0 + (lane & 3u);
+    py = y0 + ((lane >> 2) & 3u);
 }
 
 // Stages splats [base, min(base + 256, end)) of the tile's sorted list in LDS and builds one
-// compacted list per quadrant (sorted order kept) of the splats that can reach alpha >= 1/255 at
-// some pixel centre of the quadrant (a bounding-box test, then splat_touches_rect).  Returns the
-// length of this wave's list.  Ends with a barrier.
+// compacted list per 4x4 sub-block (sorted order kept) of the splats that can reach
+// alpha >= 1/255 at some pixel centre of the sub-block (a bounding-box test, then
+// splat_touches_rect).  Returns the number of list entries this wave walks: the longest of its
+// four lists (the shorter ones are padded with the sentinel).  Ends with a barrier.
 template <bool STORE_G>
 __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
                                                 uint32_t end, const uint32_t* __restrict__ vals,
                                                 const float4* __restrict__ xyzr, const float4* __restrict__ conop,
                                                 const float4* __restrict__ rgb) {
-    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    const uint32_t tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = __lane_id();
+    {
+        constexpr uint32_t s2 = kSentinel | (kSentinel << 16);
+        uint4* lw = reinterpret_cast<uint4*>(&s.list[0][0]);
+        static_assert(sizeof(s.list) == 2 * kTilePixels * sizeof(uint4), "list fill");
+        lw[tid] = make_uint4(s2, s2, s2, s2);
+        lw[tid + kTilePixels] = make_uint4(s2, s2, s2, s2);
+        if (tid == 0) {
+            s.a[kTilePixels] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            s.b[kTilePixels] = make_float4(0.0f, __builtin_inff(), 0.0f, 0.0f);
+            s.c[kTilePixels] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
     const uint32_t k = base + tid;
     uint32_t mask = 0;
     if (k < end) {
@@ -63,7 +101,7 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
         s.c[tid] = rgb[g];
         if (STORE_G) s_g[tid] = g;
         // Bounding box of the region Q <= q0 outside which splat_touches_rect always says no
-        // (half-extents sqrt(q0 c / det), sqrt(q0 a / det), widened by 1%); only quadrants the
+        // (half-extents sqrt(q0 c / det), sqrt(q0 a / det), widened by 1%); only sub-blocks the
         // box reaches get the exact test.  Degenerate or near-degenerate conics (|b| close to
         // sqrt(ac), where det loses precision) give an unbounded box.
         float ex = __builtin_inff(), ey = __builtin_inff();
@@ -73,50 +111,71 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
             ex = 1.01f * __builtin_sqrtf(q0 * c.z / det) + 1e-3f;
             ey = 1.01f * __builtin_sqrtf(q0 * c.x / det) + 1e-3f;
         }
-#pragma unroll
-        for (uint32_t q = 0; q < kBlendWaves; ++q) {
-            const uint32_t x0 = tv.tx * kTile + (q & 1u) * kQuad, y0 = tv.ty * kTile + (q >> 1) * kQuad;
-            if (x0 >= tv.width || y0 >= tv.height) continue;
-            const float fx0 = (float)x0, fy0 = (float)y0;
-            const float fx1 = (float)(min(x0 + kQuad, tv.width) - 1), fy1 = (float)(min(y0 + kQuad, tv.height) - 1);
-            if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
-                splat_touches_rect(p.x, p.y, c.x, c.y, c.z, c.w, fx0, fx1, fy0, fy1))
-                mask |= 1u << q;
+        const SplatReach r = splat_reach(p.x, p.y, c.x, c.y, c.z, c.w);
+        // columns / rows of sub-blocks (4 px each) the box overlaps, clamped to the tile in float
+        // before the conversion (infinite boxes, NaN -> 0)
+        const float ox = (float)(tv.tx * kTile), oy = (float)(tv.ty * kTile);
+        const auto col = [](float v) { return (int32_t)__builtin_floorf(fminf(fmaxf(v * 0.25f, 0.0f), 3.0f)); };
+        const int32_t cx0 = col(p.x - ex - ox), cx1 = col(p.x + ex - ox);
+        const int32_t cy0 = col(p.y - ey - oy), cy1 = col(p.y + ey - oy);
+        for (int32_t cy = cy0; cy <= cy1; ++cy) {
+            for (int32_t cx = cx0; cx <= cx1; ++cx) {
+                const uint32_t x0 = tv.tx * kTile + (uint32_t)cx * kSub, y0 = tv.ty * kTile + (uint32_t)cy * kSub;
+                if (x0 >= tv.width || y0 >= tv.height) continue;
+                const float fx0 = (float)x0, fy0 = (float)y0;
+                const float fx1 = (float)(min(x0 + kSub, tv.width) - 1), fy1 = (float)(min(y0 + kSub, tv.height) - 1);
+                if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
+                    splat_touches_rect(r, fx0, fx1, fy0, fy1)) {
+                    // sub-block index: quadrant (cx >> 1, cy >> 1), block (cx & 1, cy & 1) inside it
+                    const uint32_t q = (((uint32_t)cy >> 1) * 2u + ((uint32_t)cx >> 1)) * 4u +
+                                       ((uint32_t)cy & 1u) * 2u + ((uint32_t)cx & 1u);
+                    mask |= 1u << q;
+                }
+            }
         }
     }
     const uint64_t lt = lanemask_lt();
-    uint64_t bal[kBlendWaves];
+    uint64_t bal[kSubBlocks];
 #pragma unroll
-    for (uint32_t q = 0; q < kBlendWaves; ++q) bal[q] = ballot((mask >> q) & 1u);
-    if (lane < kBlendWaves) {
-        uint64_t b = bal[0];
+    for (uint32_t q = 0; q < kSubBlocks; ++q) bal[q] = ballot((mask >> q) & 1u);
+    if (lane == 0) {
 #pragma unroll
-        for (uint32_t q = 1; q < kBlendWaves; ++q) b = lane == q ? bal[q] : b;
-        s.cnt[wave][lane] = (uint32_t)__popcll(b);
+        for (uint32_t q = 0; q < kSubBlocks; ++q) s.cnt[wave][q] = (uint32_t)__popcll(bal[q]);
     }
     __syncthreads();
+    // lane q < 16: this wave's start in sub-block q's list, and the list's length
+    uint32_t start = 0, total = 0;
+    if (lane < kSubBlocks) {
 #pragma unroll
-    for (uint32_t q = 0; q < kBlendWaves; ++q) {
-        if ((mask >> q) & 1u) {
-            uint32_t off = (uint32_t)__popcll(bal[q] & lt);
-            for (uint32_t w = 0; w < wave; ++w) off += s.cnt[w][q];
-            s.list[q][off] = (uint8_t)tid;
+        for (uint32_t w = 0; w < kBlendWaves; ++w) {
+            const uint32_t n = s.cnt[w][lane];
+            start += w < wave ? n : 0u;
+            total += n;
         }
     }
-    uint32_t cnt = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kBlendWaves; ++w) cnt += s.cnt[w][wave];
+    for (uint32_t q = 0; q < kSubBlocks; ++q) {
+        if ((mask >> q) & 1u) {
+            const uint32_t off = (uint32_t)__builtin_amdgcn_readlane(start, q) + (uint32_t)__popcll(bal[q] & lt);
+            s.list[q][off] = (uint16_t)(tid * kSlotBytes);
+        }
+    }
+    uint32_t walk = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) walk = max(walk, (uint32_t)__builtin_amdgcn_readlane(total, wave * 4u + g));
     __syncthreads();
-    return cnt;
+    return walk;
 }
 
 // Forward composite of one tile.  Per pixel the operations and their order are the reference's;
 // the shortcuts around them change no pixel:
-//   - the quadrant lists of blend_stage drop only splats no pixel of the quadrant can take;
+//   - the sub-block lists of blend_stage drop only splats no pixel of the sub-block can take;
 //   - skip = log(1/(255 o)) - 0.01: a power below it gives o * exp(power) < 0.99/255, so the
 //     alpha < 1/255 test would discard the splat anyway; exp() runs only if some lane needs it;
 //   - a wave leaves the round as soon as all its pixels have terminated.
-// COUNT adds up the pixel-splat pairs whose power is evaluated (for the FLOP roofline).
+// Each 16-lane row of a wave walks its own sub-block's list, so one wave step evaluates up to four
+// different splats.  COUNT adds up the pixel-splat pairs whose power is evaluated (for the FLOP
+// roofline; sentinel entries are not counted).
 template <bool COUNT>
 __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint2 range,
                                            const uint32_t* __restrict__ vals, const float4* __restrict__ xyzr,
@@ -134,17 +193,16 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     uint64_t live = ballot(inside);
     uint32_t evals = 0;
-    const uint8_t* list = s.list[__builtin_amdgcn_readfirstlane(wave)];
+    const uint16_t* list = s.list[wave * 4u + (lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
-            blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb));
+        const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb);
         if (live == 0) continue;
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t j = list[i];
-            const float4 A = s.a[j];
-            const float2 B = *reinterpret_cast<const float2*>(&s.b[j]);  // conic c, skip threshold
-            if (COUNT) evals += lane_on(live) ? 1u : 0u;
+        for (uint32_t i = 0; i < walk; ++i) {
+            const uint32_t off = list[i];
+            const float4 A = slot4(s.a, off);
+            const float2 B = slot2(s.b, off);  // conic c, skip threshold
+            if (COUNT) evals += (lane_on(live) && off != kSentinel) ? 1u : 0u;
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             // power <= 0 equals !(power > 0) except for NaN, whose exp is 0 and whose alpha
@@ -152,13 +210,13 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
             // power in [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf's plain range.
             const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
             if (act == 0) continue;
-            float alpha = s.b[j].z * soft_expf_inrange(power);
+            float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const float tn = T * (1.0f - alpha);
             const uint64_t ok = act & ballot(!(alpha < 1.0f / 255.0f));
             const uint64_t low = ballot(tn < 0.0001f);
             if (lane_on(ok & ~low)) {
-                const float4 C = s.c[j];
+                const float4 C = slot4(s.c, off);
                 cr = cr + (C.x * alpha) * T;
                 cg = cg + (C.y * alpha) * T;
                 cb = cb + (C.z * alpha) * T;
@@ -230,28 +288,27 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     float T = 1.0f;
     uint64_t live = ballot(inside);
     int32_t last = -1;
-    const uint8_t* list = s.list[__builtin_amdgcn_readfirstlane(wave)];
+    const uint16_t* list = s.list[wave * 4u + (lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
-            blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb));
+        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
         if (live == 0) continue;
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t j = list[i];
-            const float4 A = s.a[j];
-            const float2 B = *reinterpret_cast<const float2*>(&s.b[j]);
+        for (uint32_t i = 0; i < walk; ++i) {
+            const uint32_t off = list[i];
+            const float4 A = slot4(s.a, off);
+            const float2 B = slot2(s.b, off);
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
             if (act == 0) continue;
-            float alpha = s.b[j].z * soft_expf_inrange(power);
+            float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const float tn = T * (1.0f - alpha);
             const uint64_t ok = act & ballot(!(alpha < 1.0f / 255.0f));
             const uint64_t low = ballot(tn < 0.0001f);
             if (lane_on(ok & ~low)) {
                 T = tn;
-                last = (int32_t)(base + j);
+                last = (int32_t)(base + off / kSlotBytes);
             }
             live &= ~(ok & low);
             if (live == 0) break;
@@ -282,15 +339,15 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     for (int32_t chunk = (int32_t)first_chunk; chunk >= 0; --chunk) {
         const uint32_t base = range.x + (uint32_t)chunk * kTilePixels;
         __syncthreads();
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(
-            blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb));
-        for (int32_t i = (int32_t)cnt - 1; i >= 0; --i) {
-            const uint32_t j = list[i];
+        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
+        for (int32_t i = (int32_t)walk - 1; i >= 0; --i) {
+            const uint32_t off = list[i];
+            const uint32_t j = off / kSlotBytes;  // 256 = the sentinel, never active
             const int32_t kk = (int32_t)(base + j);
             const uint64_t mine = ballot(kk <= last);
             if (mine == 0) continue;
-            const float4 A = s.a[j];
-            const float4 B = s.b[j];
+            const float4 A = slot4(s.a, off);
+            const float4 B = slot4(s.b, off);
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
             const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & mine;
@@ -304,7 +361,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
 #pragma unroll
             for (uint32_t q = 0; q < kG; ++q) v[q] = 0.0f;
             if (upd) {
-                const float4 C = s.c[j];
+                const float4 C = slot4(s.c, off);
                 T = T / (1.0f - alpha);  // transmittance in front of this splat
                 acc_r = last_alpha * last_r + (1.0f - last_alpha) * acc_r;
                 acc_g = last_alpha * last_g + (1.0f - last_alpha) * acc_g;
@@ -331,7 +388,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
 #pragma unroll
                 for (uint32_t q = 0; q < kG; ++q) {
                     const float sum = row_sum16(v[q]);
-                    if ((lane & 15u) == 15u && sum != 0.0f) atomicAdd(&L.acc[j][q], sum);
+                    if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], sum);
                 }
             }
         }

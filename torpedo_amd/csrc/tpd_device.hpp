@@ -293,34 +293,54 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
 // relative, so every pixel's power is then below -ln(255 o) - 0.04 and o * exp(power) < 1/255
 // (exp is accurate to 2.5 ulp).  Anything else (NaN, non-positive-definite conic, opacity that
 // may reach 1/255) is kept.
-__device__ __forceinline__ bool splat_touches_rect(float gx, float gy, float a, float b, float c, float o, float x0,
-                                                   float x1, float y0, float y1) {
-    if (o < 1.0f / 255.0f) return false;  // alpha <= o for power <= 0
-    if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c))) return true;
-    if (gx >= x0 && gx <= x1 && gy >= y0 && gy <= y1) return true;
+//
+// SplatReach holds the per-splat parts of the test so that one splat can be tested against many
+// rectangles.  The edge minimisers use -b/c and -b/a rounded once: any point of an edge bounds
+// the edge minimum from above only by c (or a) times the squared minimiser error (relative 1e-13
+// of Q), far inside the 1 % margin.
+struct SplatReach {
+    float gx, gy, a, b, c;
+    float mbc, mba;  // -b / c, -b / a
+    float thr;       // ln(255 o) + 0.05
+    bool never, always;
+};
+
+__device__ __forceinline__ SplatReach splat_reach(float gx, float gy, float a, float b, float c, float o) {
+    SplatReach r;
+    r.gx = gx; r.gy = gy; r.a = a; r.b = b; r.c = c;
+    r.never = o < 1.0f / 255.0f;  // alpha <= o for power <= 0
+    r.always = !(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c));
+    r.mbc = -b * __builtin_amdgcn_rcpf(c);
+    r.mba = -b * __builtin_amdgcn_rcpf(a);
+    r.thr = __logf(255.0f * o) + 0.05f;
+    return r;
+}
+
+__device__ __forceinline__ bool splat_touches_rect(const SplatReach& r, float x0, float x1, float y0, float y1) {
+    if (r.never) return false;
+    if (r.always) return true;
+    if (r.gx >= x0 && r.gx <= x1 && r.gy >= y0 && r.gy <= y1) return true;
     float q = __builtin_inff();
     {
         // vertical edges X = x0, x1: dy in [gy - y1, gy - y0]
-        const float lo = gy - y1, hi = gy - y0;
+        const float lo = r.gy - y1, hi = r.gy - y0;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float dx = gx - (e ? x1 : x0);
-            float dy = -(b * dx) / c;
-            dy = fminf(fmaxf(dy, lo), hi);
-            q = fminf(q, (a * dx) * dx + 2.0f * (b * dx) * dy + (c * dy) * dy);
+            const float dx = r.gx - (e ? x1 : x0);
+            const float dy = fminf(fmaxf(r.mbc * dx, lo), hi);
+            q = fminf(q, (r.a * dx) * dx + 2.0f * (r.b * dx) * dy + (r.c * dy) * dy);
         }
     }
     {
-        const float lo = gx - x1, hi = gx - x0;
+        const float lo = r.gx - x1, hi = r.gx - x0;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float dy = gy - (e ? y1 : y0);
-            float dx = -(b * dy) / a;
-            dx = fminf(fmaxf(dx, lo), hi);
-            q = fminf(q, (a * dx) * dx + 2.0f * (b * dx) * dy + (c * dy) * dy);
+            const float dy = r.gy - (e ? y1 : y0);
+            const float dx = fminf(fmaxf(r.mba * dy, lo), hi);
+            q = fminf(q, (r.a * dx) * dx + 2.0f * (r.b * dx) * dy + (r.c * dy) * dy);
         }
     }
-    return !(0.99f * 0.5f * q > __logf(255.0f * o) + 0.05f);
+    return !(0.99f * 0.5f * q > r.thr);
 }
 
 // ------------------------------------------------------------------ wave64 helpers
