@@ -50,28 +50,33 @@ __device__ __forceinline__ float2 slot2(const float4* arr, uint32_t off) {
     return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(arr) + off);
 }
 
-// Sub-block q of the tile (0..15): quadrant q >> 2 (wave), 4x4 block q & 3 inside it.
+// Sub-blocks are numbered row-major inside the tile: q = 4 * row + column (4x4 pixels each).
 __device__ __forceinline__ void sub_block_origin(const TileView& tv, uint32_t q, uint32_t& x0, uint32_t& y0) {
-    const uint32_t w = q >> 2, g = q & 3u;
-    x0 = tv.tx * kTile + (w & 1u) * kQuad + (g & 1u) * kSub;
-    y0 = tv.ty * kTile + (w >> 1) * kQuad + (g >> 1) * kSub;
+    x0 = tv.tx * kTile + (q & 3u) * kSub;
+    y0 = tv.ty * kTile + (q >> 2) * kSub;
 }
 
-// Pixel of this thread: wave w owns the 8x8 quadrant (w & 1, w >> 1) of the tile, the 16-lane row
-// r of the wave its 4x4 sub-block (r & 1, r >> 1), lanes in row-major order inside it.
+// Sub-block walked by row g (0..3) of wave w: wave w owns the 8x8 quadrant (w & 1, w >> 1) of the
+// tile, row g the 4x4 block (g & 1, g >> 1) inside it.
+__device__ __forceinline__ uint32_t sub_block_of(uint32_t w, uint32_t g) {
+    return (((w >> 1) * 2u + (g >> 1)) * 4u) + (w & 1u) * 2u + (g & 1u);
+}
+
+// Pixel of this thread: the sub-block of its wave and 16-lane row, lanes row-major inside it.
 __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uint32_t& py) {
     const uint32_t wave = threadIdx.x / kWave, lane = __lane_id();
     uint32_t x0, y0;
-    sub_block_origin(tv, wave * 4u + (lane >> 4), x0, y0);
+    sub_block_origin(tv, sub_block_of(wave, lane >> 4), x0, y0);
     px = x0 + (lane & 3u);
     py = y0 + ((lane >> 2) & 3u);
 }
 
 // Stages splats [base, min(base + 256, end)) of the tile's sorted list in LDS and builds one
-// compacted list per 4x4 sub-block (sorted order kept) of the splats that can reach
-// alpha >= 1/255 at some pixel centre of the sub-block (a bounding-box test, then
-// splat_touches_rect).  Returns the number of list entries this wave walks: the longest of its
-// four lists (the shorter ones are padded with the sentinel).  Ends with a barrier.
+// compacted list per 4x4 sub-block (sorted order kept) of the splats that may reach
+// alpha >= 1/255 at some pixel centre of the sub-block: the splat passes the exact test
+// (splat_touches_rect) for the sub-block's 8x8 quadrant and the bounding box of its alpha >= 1/255
+// region overlaps the sub-block.  Returns the number of list entries this wave walks: the longest
+// of its four lists (the shorter ones are padded with the sentinel).  Ends with a barrier.
 template <bool STORE_G>
 __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
                                                 uint32_t end, const uint32_t* __restrict__ vals,
@@ -111,28 +116,36 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
             ex = 1.01f * __builtin_sqrtf(q0 * c.z / det) + 1e-3f;
             ey = 1.01f * __builtin_sqrtf(q0 * c.x / det) + 1e-3f;
         }
+        // Exact test per 8x8 quadrant, then the box per 4x4 sub-block inside the quadrants that
+        // pass.  Sub-block bits are row-major (bit 4 * row + column).
         const SplatReach r = splat_reach(p.x, p.y, c.x, c.y, c.z, c.w);
-        // columns / rows of sub-blocks (4 px each) the box overlaps, clamped to the tile in float
-        // before the conversion (infinite boxes, NaN -> 0)
-        const float ox = (float)(tv.tx * kTile), oy = (float)(tv.ty * kTile);
-        const auto col = [](float v) { return (int32_t)__builtin_floorf(fminf(fmaxf(v * 0.25f, 0.0f), 3.0f)); };
-        const int32_t cx0 = col(p.x - ex - ox), cx1 = col(p.x + ex - ox);
-        const int32_t cy0 = col(p.y - ey - oy), cy1 = col(p.y + ey - oy);
-        for (int32_t cy = cy0; cy <= cy1; ++cy) {
-            for (int32_t cx = cx0; cx <= cx1; ++cx) {
-                const uint32_t x0 = tv.tx * kTile + (uint32_t)cx * kSub, y0 = tv.ty * kTile + (uint32_t)cy * kSub;
-                if (x0 >= tv.width || y0 >= tv.height) continue;
-                const float fx0 = (float)x0, fy0 = (float)y0;
-                const float fx1 = (float)(min(x0 + kSub, tv.width) - 1), fy1 = (float)(min(y0 + kSub, tv.height) - 1);
-                if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
-                    splat_touches_rect(r, fx0, fx1, fy0, fy1)) {
-                    // sub-block index: quadrant (cx >> 1, cy >> 1), block (cx & 1, cy & 1) inside it
-                    const uint32_t q = (((uint32_t)cy >> 1) * 2u + ((uint32_t)cx >> 1)) * 4u +
-                                       ((uint32_t)cy & 1u) * 2u + ((uint32_t)cx & 1u);
-                    mask |= 1u << q;
-                }
-            }
+        uint32_t quads = 0;
+#pragma unroll
+        for (uint32_t qd = 0; qd < kBlendWaves; ++qd) {
+            const uint32_t x0 = tv.tx * kTile + (qd & 1u) * kQuad, y0 = tv.ty * kTile + (qd >> 1) * kQuad;
+            if (x0 >= tv.width || y0 >= tv.height) continue;
+            const float fx0 = (float)x0, fy0 = (float)y0;
+            const float fx1 = (float)(min(x0 + kQuad, tv.width) - 1), fy1 = (float)(min(y0 + kQuad, tv.height) - 1);
+            if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
+                splat_touches_rect(r, fx0, fx1, fy0, fy1))
+                quads |= 0x33u << ((qd & 1u) * 2u + (qd >> 1) * 8u);
         }
+        // columns / rows of sub-blocks the box overlaps, clamped to the tile in float before the
+        // conversion (infinite boxes, NaN -> 0)
+        const float ox = (float)(tv.tx * kTile), oy = (float)(tv.ty * kTile);
+        const auto col = [](float v) { return (uint32_t)__builtin_floorf(fminf(fmaxf(v * 0.25f, 0.0f), 3.0f)); };
+        const uint32_t cx0 = col(p.x - ex - ox), cx1 = col(p.x + ex - ox);
+        const uint32_t cy0 = col(p.y - ey - oy), cy1 = col(p.y + ey - oy);
+        const uint32_t cols = ((2u << cx1) - 1u) & ~((1u << cx0) - 1u);
+        const uint32_t rows = ((2u << cy1) - 1u) & ~((1u << cy0) - 1u);
+        const uint32_t spread = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
+        // sub-blocks with at least one pixel inside the view
+        const uint32_t vx = min((tv.width - tv.tx * kTile + kSub - 1) / kSub, 4u);
+        const uint32_t vy = min((tv.height - tv.ty * kTile + kSub - 1) / kSub, 4u);
+        const uint32_t vcols = (1u << vx) - 1u;
+        const uint32_t vrows = (1u << vy) - 1u;
+        const uint32_t vspread = (vrows & 1u) | ((vrows & 2u) << 3) | ((vrows & 4u) << 6) | ((vrows & 8u) << 9);
+        mask = quads & (cols * spread) & (vcols * vspread);
     }
     const uint64_t lt = lanemask_lt();
     uint64_t bal[kSubBlocks];
@@ -162,7 +175,7 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
     }
     uint32_t walk = 0;
 #pragma unroll
-    for (uint32_t g = 0; g < 4; ++g) walk = max(walk, (uint32_t)__builtin_amdgcn_readlane(total, wave * 4u + g));
+    for (uint32_t g = 0; g < 4; ++g) walk = max(walk, (uint32_t)__builtin_amdgcn_readlane(total, sub_block_of(wave, g)));
     __syncthreads();
     return walk;
 }
@@ -193,7 +206,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     uint64_t live = ballot(inside);
     uint32_t evals = 0;
-    const uint16_t* list = s.list[wave * 4u + (lane >> 4)];
+    const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
         const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb);
@@ -288,7 +301,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     float T = 1.0f;
     uint64_t live = ballot(inside);
     int32_t last = -1;
-    const uint16_t* list = s.list[wave * 4u + (lane >> 4)];
+    const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
         const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
