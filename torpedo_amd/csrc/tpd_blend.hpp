@@ -73,10 +73,9 @@ __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uin
 
 // Stages splats [base, min(base + 256, end)) of the tile's sorted list in LDS and builds one
 // compacted list per 4x4 sub-block (sorted order kept) of the splats that may reach
-// alpha >= 1/255 at some pixel centre of the sub-block: the splat passes the exact test
-// (splat_touches_rect) for the sub-block's 8x8 quadrant and the bounding box of its alpha >= 1/255
-// region overlaps the sub-block.  Returns the number of list entries this wave walks: the longest
-// of its four lists (the shorter ones are padded with the sentinel).  Ends with a barrier.
+// alpha >= 1/255 at some pixel centre of the sub-block (splat_sub_block_mask).  Returns the number
+// of list entries this wave walks: the longest of its four lists (the shorter ones are padded
+// with the sentinel).  Ends with a barrier.
 template <bool STORE_G>
 __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
                                                 uint32_t end, const uint32_t* __restrict__ vals,
@@ -105,47 +104,15 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
         s.b[tid] = make_float4(c.z, __logf(1.0f / (255.0f * c.w)) - 0.01f, c.w, 0.0f);
         s.c[tid] = rgb[g];
         if (STORE_G) s_g[tid] = g;
-        // Bounding box of the region Q <= q0 outside which splat_touches_rect always says no
-        // (half-extents sqrt(q0 c / det), sqrt(q0 a / det), widened by 1%); only sub-blocks the
-        // box reaches get the exact test.  Degenerate or near-degenerate conics (|b| close to
-        // sqrt(ac), where det loses precision) give an unbounded box.
-        float ex = __builtin_inff(), ey = __builtin_inff();
-        const float det = c.x * c.z - c.y * c.y;
-        const float q0 = (__logf(255.0f * c.w) + 0.05f) * (2.0f / 0.99f);
-        if (c.x > 0.0f && c.z > 0.0f && c.y * c.y < 0.98f * (c.x * c.z) && q0 >= 0.0f) {
-            ex = 1.01f * __builtin_sqrtf(q0 * c.z / det) + 1e-3f;
-            ey = 1.01f * __builtin_sqrtf(q0 * c.x / det) + 1e-3f;
-        }
-        // Exact test per 8x8 quadrant, then the box per 4x4 sub-block inside the quadrants that
-        // pass.  Sub-block bits are row-major (bit 4 * row + column).
-        const SplatReach r = splat_reach(p.x, p.y, c.x, c.y, c.z, c.w);
-        uint32_t quads = 0;
-#pragma unroll
-        for (uint32_t qd = 0; qd < kBlendWaves; ++qd) {
-            const uint32_t x0 = tv.tx * kTile + (qd & 1u) * kQuad, y0 = tv.ty * kTile + (qd >> 1) * kQuad;
-            if (x0 >= tv.width || y0 >= tv.height) continue;
-            const float fx0 = (float)x0, fy0 = (float)y0;
-            const float fx1 = (float)(min(x0 + kQuad, tv.width) - 1), fy1 = (float)(min(y0 + kQuad, tv.height) - 1);
-            if (p.x + ex >= fx0 && p.x - ex <= fx1 && p.y + ey >= fy0 && p.y - ey <= fy1 &&
-                splat_touches_rect(r, fx0, fx1, fy0, fy1))
-                quads |= 0x33u << ((qd & 1u) * 2u + (qd >> 1) * 8u);
-        }
-        // columns / rows of sub-blocks the box overlaps, clamped to the tile in float before the
-        // conversion (infinite boxes, NaN -> 0)
-        const float ox = (float)(tv.tx * kTile), oy = (float)(tv.ty * kTile);
-        const auto col = [](float v) { return (uint32_t)__builtin_floorf(fminf(fmaxf(v * 0.25f, 0.0f), 3.0f)); };
-        const uint32_t cx0 = col(p.x - ex - ox), cx1 = col(p.x + ex - ox);
-        const uint32_t cy0 = col(p.y - ey - oy), cy1 = col(p.y + ey - oy);
-        const uint32_t cols = ((2u << cx1) - 1u) & ~((1u << cx0) - 1u);
-        const uint32_t rows = ((2u << cy1) - 1u) & ~((1u << cy0) - 1u);
-        const uint32_t spread = (rows & 1u) | ((rows & 2u) << 3) | ((rows & 4u) << 6) | ((rows & 8u) << 9);
+        const uint32_t reach = splat_sub_block_mask(p.x, p.y, c.x, c.y, c.z, c.w, (float)(tv.tx * kTile),
+                                                    (float)(tv.ty * kTile));
         // sub-blocks with at least one pixel inside the view
         const uint32_t vx = min((tv.width - tv.tx * kTile + kSub - 1) / kSub, 4u);
         const uint32_t vy = min((tv.height - tv.ty * kTile + kSub - 1) / kSub, 4u);
         const uint32_t vcols = (1u << vx) - 1u;
         const uint32_t vrows = (1u << vy) - 1u;
         const uint32_t vspread = (vrows & 1u) | ((vrows & 2u) << 3) | ((vrows & 4u) << 6) | ((vrows & 8u) << 9);
-        mask = quads & (cols * spread) & (vcols * vspread);
+        mask = reach & (vcols * vspread);
     }
     const uint64_t lt = lanemask_lt();
     uint64_t bal[kSubBlocks];

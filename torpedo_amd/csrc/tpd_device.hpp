@@ -282,65 +282,47 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
     return true;
 }
 
-// Whether a splat can pass the blend's alpha >= 1/255 test at any pixel centre of the rectangle
-// [x0, x1] x [y0, y1].  Returns false only when every pixel of the rectangle is certain to skip it,
-// so dropping such splats from a tile's list leaves every pixel bit-identical.
+// Which 4x4-pixel sub-blocks of the 16x16 tile with top-left pixel (ox, oy) a splat may reach:
+// bit 4 * row + column is set unless no pixel centre of that sub-block can pass the blend's
+// alpha >= 1/255 test, so dropping the splat from the other sub-blocks' lists leaves every pixel
+// bit-identical.
 //
-// power = -Q/2 with Q(d) = a dx^2 + 2 b dx dy + c dy^2, d = centre - pixel; the minimum of the
-// convex Q over the rectangle is 0 if the centre lies inside, else the smallest of the four edge
-// minima (each a clamped 1-D quadratic).  Skip when 0.99 * Qmin/2 > ln(255 o) + 0.05: with
-// |b| < 0.99 sqrt(ac) the float error of Q and of the per-pixel power stays below 2e-4
-// relative, so every pixel's power is then below -ln(255 o) - 0.04 and o * exp(power) < 1/255
-// (exp is accurate to 2.5 ulp).  Anything else (NaN, non-positive-definite conic, opacity that
-// may reach 1/255) is kept.
-//
-// SplatReach holds the per-splat parts of the test so that one splat can be tested against many
-// rectangles.  The edge minimisers use -b/c and -b/a rounded once: any point of an edge bounds
-// the edge minimum from above only by c (or a) times the squared minimiser error (relative 1e-13
-// of Q), far inside the 1 % margin.
-struct SplatReach {
-    float gx, gy, a, b, c;
-    float mbc, mba;  // -b / c, -b / a
-    float thr;       // ln(255 o) + 0.05
-    bool never, always;
-};
-
-__device__ __forceinline__ SplatReach splat_reach(float gx, float gy, float a, float b, float c, float o) {
-    SplatReach r;
-    r.gx = gx; r.gy = gy; r.a = a; r.b = b; r.c = c;
-    r.never = o < 1.0f / 255.0f;  // alpha <= o for power <= 0
-    r.always = !(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c));
-    r.mbc = -b * __builtin_amdgcn_rcpf(c);
-    r.mba = -b * __builtin_amdgcn_rcpf(a);
-    r.thr = __logf(255.0f * o) + 0.05f;
-    return r;
-}
-
-__device__ __forceinline__ bool splat_touches_rect(const SplatReach& r, float x0, float x1, float y0, float y1) {
-    if (r.never) return false;
-    if (r.always) return true;
-    if (r.gx >= x0 && r.gx <= x1 && r.gy >= y0 && r.gy <= y1) return true;
-    float q = __builtin_inff();
-    {
-        // vertical edges X = x0, x1: dy in [gy - y1, gy - y0]
-        const float lo = r.gy - y1, hi = r.gy - y0;
+// power = -Q/2 with Q(u, v) = a u^2 + 2 b u v + c v^2, (u, v) = pixel - centre.  A pixel is
+// certainly skipped when 0.99 * Q/2 > ln(255 o) + 0.05: with |b| < 0.99 sqrt(ac) the float error
+// of the per-pixel power stays below 2e-4 relative, so its power is then below -ln(255 o) - 0.04
+// and o * exp(power) < 1/255 (exp is accurate to 2.5 ulp).  The mask covers the region
+// Q <= q0 = 1.02 * (ln(255 o) + 0.05) * 2 / 0.99, that bound inflated by 2 %.  For each band of
+// four pixel rows the region's x-extent has a closed form: with det = ac - b^2 the chord at height
+// v is u = (-b v +- sqrt(a q0 - det v^2)) / a, its right end is largest at the rightmost point of
+// the ellipse, v* = -b ex / c (ex = sqrt(q0 c / det)), and its left end smallest at -v*; clamping
+// v* (and -v*) into the band gives the band's extent.  A slack of 0.01 px + 2^-20 |coordinate|
+// covers the rounding of these few operations.  Opacity below 1/255 reaches nothing; a conic that
+// is not safely positive definite (b^2 >= 0.98 ac, NaN) may reach everything.
+__host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, float a, float b, float c, float o,
+                                                         float ox, float oy) {
+    if (o < 1.0f / 255.0f) return 0u;  // alpha <= o for power <= 0
+    const float q0 = 1.02f * (logf(255.0f * o) + 0.05f) * (2.0f / 0.99f);
+    if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c) && q0 >= 0.0f)) return 0xFFFFu;
+    const float det = a * c - b * b;
+    const float ex = sqrtf(q0 * c / det), ey = sqrtf(q0 * a / det);
+    const float vstar = -b * ex / c;
+    const float aq0 = a * q0, inva = 1.0f / a;
+    const float sx = 0.01f + fabsf(gx) * (1.0f / 1048576.0f), sy = 0.01f + fabsf(gy) * (1.0f / 1048576.0f);
+    uint32_t mask = 0;
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const float dx = r.gx - (e ? x1 : x0);
-            const float dy = fminf(fmaxf(r.mbc * dx, lo), hi);
-            q = fminf(q, (r.a * dx) * dx + 2.0f * (r.b * dx) * dy + (r.c * dy) * dy);
-        }
+    for (uint32_t r = 0; r < 4; ++r) {
+        const float lo = (oy + 4.0f * (float)r) - gy, hi = lo + 3.0f;
+        if (hi < -ey - sy || lo > ey + sy) continue;
+        const float vlo = fmaxf(lo, -ey), vhi = fminf(hi, ey);
+        const float vr = fminf(fmaxf(vstar, vlo), vhi), vl = fminf(fmaxf(-vstar, vlo), vhi);
+        const float xr = gx + (-b * vr + sqrtf(fmaxf(aq0 - det * vr * vr, 0.0f))) * inva + sx;
+        const float xl = gx + (-b * vl - sqrtf(fmaxf(aq0 - det * vl * vl, 0.0f))) * inva - sx;
+        if (xr < ox || xl > ox + 15.0f) continue;
+        const uint32_t c0 = (uint32_t)floorf(fminf(fmaxf((xl - ox) * 0.25f, 0.0f), 3.0f));
+        const uint32_t c1 = (uint32_t)floorf(fminf(fmaxf((xr - ox) * 0.25f, 0.0f), 3.0f));
+        mask |= (((2u << c1) - 1u) & ~((1u << c0) - 1u)) << (4u * r);
     }
-    {
-        const float lo = r.gx - x1, hi = r.gx - x0;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const float dy = r.gy - (e ? y1 : y0);
-            const float dx = fminf(fmaxf(r.mba * dy, lo), hi);
-            q = fminf(q, (r.a * dx) * dx + 2.0f * (r.b * dx) * dy + (r.c * dy) * dy);
-        }
-    }
-    return !(0.99f * 0.5f * q > r.thr);
+    return mask;
 }
 
 // ------------------------------------------------------------------ wave64 helpers
