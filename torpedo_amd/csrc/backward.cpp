@@ -68,9 +68,7 @@ extern "C" int tpd_backward(tpd_ctx* c, const float* d_dL_dimage, float* d_grad_
         BackwardArgs a{};
         a.ranges = c->d_ranges;
         a.vals = c->sorted_vals;
-        a.xyzr = c->d_xyzr;
-        a.conop = c->d_conop;
-        a.rgb = c->d_rgb;
+        a.splats = c->d_splats;
         a.width = W;
         a.height = H;
         a.gx = gx;

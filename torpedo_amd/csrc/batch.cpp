@@ -50,9 +50,7 @@ struct BatchState {
     uint32_t* h_counters = nullptr;  // pinned, written by the frame's last copy
 
     // per item / tile / pixel
-    float4* d_xyzr = nullptr; size_t cap_xyzr = 0;
-    float4* d_conop = nullptr; size_t cap_conop = 0;
-    float4* d_rgb = nullptr; size_t cap_rgb = 0;
+    Splat* d_splats = nullptr; size_t cap_splats = 0;
     uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
     uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
@@ -115,7 +113,7 @@ void batch_release(tpd_ctx* c) {
     release(b->d_tile_view); release(b->d_mats); release(b->d_bases); release(b->d_scan_desc);
     release(b->d_counters);
     if (b->h_counters) (void)hipHostFree(b->h_counters);
-    release(b->d_xyzr); release(b->d_conop); release(b->d_rgb); release(b->d_tiles); release(b->d_offsets);
+    release(b->d_splats); release(b->d_tiles); release(b->d_offsets);
     release(b->d_ranges); release(b->d_image); release(b->d_rgba8);
     release(b->d_keys); release(b->d_vals); release(b->d_keys_alt); release(b->d_vals_alt);
     release(b->ws.desc); release(b->ws.partials); release(b->ws.offsets); release(b->ws.tickets);
@@ -225,8 +223,7 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_tile_view, b->cap_tile_view, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_mats, b->cap_mats, std::max<size_t>(mats, 1))) != hipSuccess ||
         (e = grow(b->d_bases, b->cap_bases, std::max<size_t>(entries, 1))) != hipSuccess ||
-        (e = grow(b->d_xyzr, b->cap_xyzr, it)) != hipSuccess || (e = grow(b->d_conop, b->cap_conop, it)) != hipSuccess ||
-        (e = grow(b->d_rgb, b->cap_rgb, it)) != hipSuccess || (e = grow(b->d_tiles, b->cap_tiles, it)) != hipSuccess ||
+        (e = grow(b->d_splats, b->cap_splats, it)) != hipSuccess || (e = grow(b->d_tiles, b->cap_tiles, it)) != hipSuccess ||
         (e = grow(b->d_offsets, b->cap_offsets, it)) != hipSuccess ||
         (e = grow(b->d_ranges, b->cap_ranges, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_image, b->cap_image, std::max<size_t>(3 * pix, 1))) != hipSuccess ||
@@ -328,9 +325,7 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     a.scan_desc = b->d_scan_desc;
     a.counters = b->d_counters;
     a.cap = b->cap;
-    a.xyzr = b->d_xyzr;
-    a.conop = b->d_conop;
-    a.rgb = b->d_rgb;
+    a.splats = b->d_splats;
     a.tiles = keep_bwd ? b->d_tiles : nullptr;
     a.offsets = keep_bwd ? b->d_offsets : nullptr;
     a.keys = b->d_keys;
@@ -350,9 +345,7 @@ static BatchBlendArgs blend_args(BatchState* b) {
     a.tile_view = b->d_tile_view;
     a.ranges = b->d_ranges;
     a.vals = b->sorted_vals;
-    a.xyzr = b->d_xyzr;
-    a.conop = b->d_conop;
-    a.rgb = b->d_rgb;
+    a.splats = b->d_splats;
     a.ntiles = b->ntiles;
     a.counters = b->d_counters;
     a.image = b->d_image;

@@ -37,9 +37,7 @@ uint32_t bit_length(uint32_t v) {
 int ensure_scene_buffers(tpd_ctx* c) {
     const size_t n = std::max<uint32_t>(c->n, 1u);
     hipError_t e;
-    if ((e = grow(c->d_xyzr, c->cap_xyzr, n)) != hipSuccess) return hip_fail(e, "alloc splats");
-    if ((e = grow(c->d_conop, c->cap_conop, n)) != hipSuccess) return hip_fail(e, "alloc splats");
-    if ((e = grow(c->d_rgb, c->cap_rgb, n)) != hipSuccess) return hip_fail(e, "alloc splats");
+    if ((e = grow(c->d_splats, c->cap_splats, n)) != hipSuccess) return hip_fail(e, "alloc splats");
     if ((e = grow(c->d_tiles, c->cap_tiles, n)) != hipSuccess) return hip_fail(e, "alloc tiles");
     if ((e = grow(c->d_offsets, c->cap_offsets, n)) != hipSuccess) return hip_fail(e, "alloc offsets");
     const size_t nd = (n + kPreBlock - 1) / kPreBlock;
@@ -168,9 +166,7 @@ int tpd_destroy(tpd_ctx* c) {
     release(c->d_ent_idx);
     release(c->d_models);
     release(c->d_ent_mats);
-    release(c->d_xyzr);
-    release(c->d_conop);
-    release(c->d_rgb);
+    release(c->d_splats);
     release(c->d_tiles);
     release(c->d_offsets);
     release(c->d_scan_desc);
@@ -336,19 +332,17 @@ int tpd_read_splats(tpd_ctx* c, float* out) {
     if (!c->rendered) return fail(TPD_ERR_STATE, "tpd_read_splats: nothing rendered");
     const size_t n = c->n;
     if (n == 0) return TPD_OK;
-    std::vector<float4> xyzr(n), conop(n), rgb(n);
+    std::vector<tpd::Splat> sp(n);
     std::vector<uint32_t> tiles(n);
     int rc;
-    if ((rc = copy_back(c, xyzr.data(), c->d_xyzr, n * sizeof(float4))) != TPD_OK) return rc;
-    if ((rc = copy_back(c, conop.data(), c->d_conop, n * sizeof(float4))) != TPD_OK) return rc;
-    if ((rc = copy_back(c, rgb.data(), c->d_rgb, n * sizeof(float4))) != TPD_OK) return rc;
+    if ((rc = copy_back(c, sp.data(), c->d_splats, n * sizeof(tpd::Splat))) != TPD_OK) return rc;
     if ((rc = copy_back(c, tiles.data(), c->d_tiles, n * sizeof(uint32_t))) != TPD_OK) return rc;
     for (size_t i = 0; i < n; ++i) {
         float* o = out + 12 * i;
-        o[0] = rgb[i].x; o[1] = rgb[i].y; o[2] = rgb[i].z;
+        o[0] = sp[i].rgb.x; o[1] = sp[i].rgb.y; o[2] = sp[i].rgb.z;
         std::memcpy(&o[3], &tiles[i], 4);
-        o[4] = xyzr[i].x; o[5] = xyzr[i].y; o[6] = xyzr[i].z; o[7] = xyzr[i].w;
-        o[8] = conop[i].x; o[9] = conop[i].y; o[10] = conop[i].z; o[11] = conop[i].w;
+        o[4] = sp[i].xyzr.x; o[5] = sp[i].xyzr.y; o[6] = sp[i].xyzr.z; o[7] = sp[i].xyzr.w;
+        o[8] = sp[i].conop.x; o[9] = sp[i].conop.y; o[10] = sp[i].conop.z; o[11] = sp[i].conop.w;
     }
     return TPD_OK;
 }

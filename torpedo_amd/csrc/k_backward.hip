@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kTilePixels) void k_blend_bwd(const BackwardArgs a)
     __shared__ BlendBwdLds lds;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv{tile % a.gx, tile / a.gx, a.width, a.height, a.gx, a.gy, 0};
-    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.dL_dimg, a.offsets, a.partials);
+    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials);
 }
 
 void launch_blend_bwd(const BackwardArgs& a, hipStream_t s) {

@@ -194,11 +194,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
         Projected p;
         p.tiles = 0;
         if (valid) {
-            if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) {
-                a.xyzr[item] = make_float4(p.px, p.py, p.vz, p.radius);
-                a.conop[item] = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
-                a.rgb[item] = make_float4(p.r, p.g, p.b, 0.0f);
-            }
+            if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) store_splat(a.splats + item, true, p);
             if (a.tiles) a.tiles[item] = p.tiles;
         }
         // block-wide inclusive scan of tile counts
@@ -251,7 +247,7 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArg
     if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
-    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.image, a.rgba8, nullptr);
+    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, nullptr);
 }
 
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s) {
@@ -266,7 +262,7 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlen
     if (a.counters[kCtrOverflow]) return;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
-    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.dL_dimg, a.offsets, a.partials);
+    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials);
 }
 
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s) {

@@ -54,15 +54,7 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
                                           (0.5f * (float)a.cam.width) * a.cam.focal_ndc[0],
                                           (0.5f * (float)a.cam.height) * a.cam.focal_ndc[1], a.cam.width,
                                           a.cam.height, a.gx, a.gy, a.sh_degree, p);
-        if (vis) {
-            a.xyzr[i] = make_float4(p.px, p.py, p.vz, p.radius);
-            a.conop[i] = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
-            a.rgb[i] = make_float4(p.r, p.g, p.b, 0.0f);
-        } else {
-            a.xyzr[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.conop[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.rgb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        store_splat(a.splats + i, vis, p);
         a.tiles[i] = p.tiles;
     }
 
@@ -163,7 +155,7 @@ __global__ __launch_bounds__(kTilePixels) void k_blend(const BlendArgs a) {
     __shared__ BlendLds lds;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv{tile % a.gx, tile / a.gx, a.width, a.height, a.gx, 0, 0};
-    blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.xyzr, a.conop, a.rgb, a.image, a.rgba8, a.eval_count);
+    blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, a.eval_count);
 }
 
 void launch_blend(const BlendArgs& a, hipStream_t s) {

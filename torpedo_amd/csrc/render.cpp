@@ -56,9 +56,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
         a.gx = gx;
         a.gy = gy;
         a.cap = (uint32_t)std::min<size_t>(c->cap_keys, kMaxSortKeys);
-        a.xyzr = c->d_xyzr;
-        a.conop = c->d_conop;
-        a.rgb = c->d_rgb;
+        a.splats = c->d_splats;
         a.tiles = c->d_tiles;
         a.offsets = c->d_offsets;
         a.scan_desc = c->d_scan_desc;
@@ -119,9 +117,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     BlendArgs b{};
     b.ranges = c->d_ranges;
     b.vals = c->sorted_vals;
-    b.xyzr = c->d_xyzr;
-    b.conop = c->d_conop;
-    b.rgb = c->d_rgb;
+    b.splats = c->d_splats;
     b.width = W;
     b.height = H;
     b.gx = gx;

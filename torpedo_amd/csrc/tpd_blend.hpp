@@ -79,8 +79,7 @@ __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uin
 template <bool STORE_G>
 __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
                                                 uint32_t end, const uint32_t* __restrict__ vals,
-                                                const float4* __restrict__ xyzr, const float4* __restrict__ conop,
-                                                const float4* __restrict__ rgb) {
+                                                const Splat* __restrict__ splats) {
     const uint32_t tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = __lane_id();
     {
         constexpr uint32_t s2 = kSentinel | (kSentinel << 16);
@@ -98,11 +97,12 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
     uint32_t mask = 0;
     if (k < end) {
         const uint32_t g = vals[k];
-        const float4 p = xyzr[g];
-        const float4 c = conop[g];
+        const Splat& sp = splats[g];
+        const float4 p = sp.xyzr;
+        const float4 c = sp.conop;
         s.a[tid] = make_float4(p.x, p.y, c.x, c.y);
-        s.b[tid] = make_float4(c.z, __logf(1.0f / (255.0f * c.w)) - 0.01f, c.w, 0.0f);
-        s.c[tid] = rgb[g];
+        s.b[tid] = make_float4(c.z, sp.aux.x, c.w, 0.0f);
+        s.c[tid] = sp.rgb;
         if (STORE_G) s_g[tid] = g;
         const uint32_t reach = splat_sub_block_mask(p.x, p.y, c.x, c.y, c.z, c.w, (float)(tv.tx * kTile),
                                                     (float)(tv.ty * kTile));
@@ -158,8 +158,7 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
 // roofline; sentinel entries are not counted).
 template <bool COUNT>
 __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint2 range,
-                                           const uint32_t* __restrict__ vals, const float4* __restrict__ xyzr,
-                                           const float4* __restrict__ conop, const float4* __restrict__ rgb,
+                                           const uint32_t* __restrict__ vals, const Splat* __restrict__ splats,
                                            float* __restrict__ image, uint8_t* __restrict__ rgba8,
                                            uint64_t* eval_count) {
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
@@ -176,7 +175,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, xyzr, conop, rgb);
+        const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, splats);
         if (live == 0) continue;
         for (uint32_t i = 0; i < walk; ++i) {
             const uint32_t off = list[i];
@@ -227,10 +226,10 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
 
 // Stores the gradient sums of the pair (this tile, splat g) at its pre-sort position; zeros when
 // out is null.
-__device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t g, const float4* __restrict__ xyzr,
+__device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t g, const Splat* __restrict__ splats,
                                                    const uint32_t* __restrict__ offsets, float* __restrict__ partials,
                                                    const float* out) {
-    const float4 p = xyzr[g];
+    const float4 p = splats[g].xyzr;
     int32_t x0, y0, x1, y1;
     bounding_rect(p.x, p.y, p.w, tv.gx, tv.gy, x0, y0, x1, y1);
     const size_t pos = (size_t)offsets[g] + (size_t)((int32_t)tv.ty - y0) * (size_t)(x1 - x0) +
@@ -250,8 +249,7 @@ __device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t 
 // of the tile's range is written (zeros behind the last contributor), so the partials need no
 // clearing beforehand.
 __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& tv, uint2 range,
-                                               const uint32_t* __restrict__ vals, const float4* __restrict__ xyzr,
-                                               const float4* __restrict__ conop, const float4* __restrict__ rgb,
+                                               const uint32_t* __restrict__ vals, const Splat* __restrict__ splats,
                                                const float* __restrict__ dL_dimg, const uint32_t* __restrict__ offsets,
                                                float* __restrict__ partials) {
     constexpr uint32_t kG = kSplatGrads;
@@ -271,7 +269,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
+        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, splats);
         if (live == 0) continue;
         for (uint32_t i = 0; i < walk; ++i) {
             const uint32_t off = list[i];
@@ -302,7 +300,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     const uint32_t zero_from = block_last < 0 ? range.x
                                               : range.x + (((uint32_t)block_last - range.x) / kTilePixels + 1u) * kTilePixels;
     for (uint32_t k = zero_from + tid; k < range.y; k += kTilePixels)
-        store_pair_partial(tv, vals[k], xyzr, offsets, partials, nullptr);
+        store_pair_partial(tv, vals[k], splats, offsets, partials, nullptr);
     if (block_last < 0) return;  // uniform
 
     // ---- pass 2: back to front
@@ -319,7 +317,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     for (int32_t chunk = (int32_t)first_chunk; chunk >= 0; --chunk) {
         const uint32_t base = range.x + (uint32_t)chunk * kTilePixels;
         __syncthreads();
-        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, xyzr, conop, rgb);
+        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, splats);
         for (int32_t i = (int32_t)walk - 1; i >= 0; --i) {
             const uint32_t off = list[i];
             const uint32_t j = off / kSlotBytes;  // 256 = the sentinel, never active
@@ -380,7 +378,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                 out[q] = L.acc[tid][q];
                 L.acc[tid][q] = 0.0f;
             }
-            store_pair_partial(tv, L.g[tid], xyzr, offsets, partials, out);
+            store_pair_partial(tv, L.g[tid], splats, offsets, partials, out);
         }
     }
 }

@@ -34,9 +34,7 @@ struct tpd_ctx {
 
     // per-Gaussian frame buffers
     tpd::EntityMats* d_ent_mats = nullptr; size_t cap_ent_mats = 0;
-    float4* d_xyzr = nullptr; size_t cap_xyzr = 0;
-    float4* d_conop = nullptr; size_t cap_conop = 0;
-    float4* d_rgb = nullptr; size_t cap_rgb = 0;
+    tpd::Splat* d_splats = nullptr; size_t cap_splats = 0;
     uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
     uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
     uint64_t* d_scan_desc = nullptr; size_t cap_scan_desc = 0;

@@ -25,6 +25,15 @@ struct CameraParams {
     uint32_t width, height;
 };
 
+// One projected splat (splat.slang:33-39 holds the same quantities), 64 bytes: the blend and its
+// backward fetch it whole, in one aligned memory segment, for each (tile, splat) pair.
+struct alignas(64) Splat {
+    float4 xyzr;   // image x, image y, view depth, radius
+    float4 conop;  // conic a, b, c, opacity
+    float4 rgb;    // colour rgb, 0
+    float4 aux;    // the blend's skip threshold log(1 / (255 opacity)) - 0.01, 0, 0, 0
+};
+
 // Per-entity products view*model and projView*model, 32 floats per entity.
 struct EntityMats {
     float vm[16];
@@ -186,6 +195,20 @@ struct Projected {
     uint32_t tiles;
     int32_t x0, y0, x1, y1;
 };
+
+// Writes one splat record: the projected values, or zeros for a culled Gaussian.
+__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p) {
+    Splat s;
+    if (vis) {
+        s.xyzr = make_float4(p.px, p.py, p.vz, p.radius);
+        s.conop = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
+        s.rgb = make_float4(p.r, p.g, p.b, 0.0f);
+        s.aux = make_float4(__logf(1.0f / (255.0f * p.opacity)) - 0.01f, 0.0f, 0.0f, 0.0f);
+    } else {
+        s.xyzr = s.conop = s.rgb = s.aux = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    *dst = s;
+}
 
 // project.slang:33-90 for the Gaussian record g (60 floats: position, opacity, quaternion xyzw,
 // scale xyz + modifier, sh[48]).  Returns false for culled / degenerate / tile-less Gaussians,

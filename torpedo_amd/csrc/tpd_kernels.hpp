@@ -20,9 +20,7 @@ struct ForwardArgs {
     CameraParams cam;
     uint32_t n, sh_degree, gx, gy;
     uint32_t cap;                // pair capacity of keys/vals
-    float4* xyzr;                // image x, image y, view depth, radius
-    float4* conop;               // conic a, b, c, opacity
-    float4* rgb;                 // colour rgb, 0
+    Splat* splats;               // per Gaussian (zeros when culled)
     uint32_t* tiles;             // tile count per Gaussian
     uint32_t* offsets;           // exclusive scan of tiles
     uint64_t* scan_desc;         // ceil(N/256) look-back descriptors, zero on entry
@@ -37,9 +35,7 @@ struct ForwardArgs {
 struct BlendArgs {
     const uint2* ranges;
     const uint32_t* vals;
-    const float4* xyzr;
-    const float4* conop;
-    const float4* rgb;
+    const Splat* splats;
     uint32_t width, height, gx, ntiles;
     float* image;            // H x W x 3 or null
     uint8_t* rgba8;          // H x W x 4 or null
@@ -49,9 +45,7 @@ struct BlendArgs {
 struct BackwardArgs {
     const uint2* ranges;
     const uint32_t* vals;
-    const float4* xyzr;
-    const float4* conop;
-    const float4* rgb;
+    const Splat* splats;
     uint32_t width, height, gx, gy, ntiles;
     const float* dL_dimg;    // H x W x 3
     const uint32_t* offsets; // per-Gaussian exclusive scan of tile counts (pre-sort pair position)
@@ -133,9 +127,7 @@ struct BatchArgs {
     uint64_t* scan_desc;          // nblocks look-back descriptors, zero on entry
     uint32_t* counters;           // kCtr* block
     uint32_t cap;                 // pair capacity
-    float4* xyzr;                 // per item (written for visible items only)
-    float4* conop;
-    float4* rgb;
+    Splat* splats;                // per item (written for visible items only)
     uint32_t* tiles;              // per item tile counts, or null (kept for the backward)
     uint32_t* offsets;            // per item pre-sort pair offsets, or null
     uint64_t* keys;
@@ -149,9 +141,7 @@ struct BatchBlendArgs {
     const uint32_t* tile_view;    // view of each global tile
     const uint2* ranges;
     const uint32_t* vals;
-    const float4* xyzr;
-    const float4* conop;
-    const float4* rgb;
+    const Splat* splats;
     uint32_t ntiles;
     const uint32_t* counters;     // the frame's kCtr* block: nothing is done after an overflow
     float* image;                 // concatenated H x W x 3 images, or null

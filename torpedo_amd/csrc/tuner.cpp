@@ -21,9 +21,7 @@ static int rerun_duplicate(tpd_ctx* c, hipStream_t s) {
     a.gx = gx;
     a.gy = gy;
     a.cap = (uint32_t)std::min<size_t>(c->cap_keys, kMaxSortKeys);
-    a.xyzr = c->d_xyzr;
-    a.conop = c->d_conop;
-    a.rgb = c->d_rgb;
+    a.splats = c->d_splats;
     a.tiles = c->d_tiles;
     a.offsets = c->d_offsets;
     a.scan_desc = c->d_scan_desc;
