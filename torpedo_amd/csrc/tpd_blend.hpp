@@ -177,18 +177,17 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
         const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, splats);
         if (live == 0) continue;
-        for (uint32_t i = 0; i < walk; ++i) {
-            const uint32_t off = list[i];
-            const float4 A = slot4(s.a, off);
-            const float2 B = slot2(s.b, off);  // conic c, skip threshold
-            if (COUNT) evals += (lane_on(live) && off != kSentinel) ? 1u : 0u;
+        // Two list entries per step: both splats' powers are formed before the first one updates
+        // the pixel (power does not depend on T), then the two updates run in list order.  An odd
+        // walk reads one sentinel entry (every list is padded with it), which no lane takes.
+        const auto power_of = [&](const float4& A, const float2& B) {
             const float dx = A.x - fx, dy = A.y - fy;
-            const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            // power <= 0 equals !(power > 0) except for NaN, whose exp is 0 and whose alpha
-            // therefore fails the 1/255 test: either way the splat is skipped.  Active lanes have
-            // power in [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf's plain range.
-            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
-            if (act == 0) continue;
+            return -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
+        };
+        // power <= 0 equals !(power > 0) except for NaN, whose exp is 0 and whose alpha therefore
+        // fails the 1/255 test: either way the splat is skipped.  Active lanes have power in
+        // [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf's plain range.
+        const auto blend_one = [&](uint32_t off, float power, uint64_t act) {
             float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const float tn = T * (1.0f - alpha);
@@ -202,6 +201,19 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
                 T = tn;
             }
             live &= ~(ok & low);
+        };
+        for (uint32_t i = 0; i < walk; i += 2) {
+            const uint32_t o1 = list[i], o2 = list[i + 1];
+            const float4 A1 = slot4(s.a, o1), A2 = slot4(s.a, o2);
+            const float2 B1 = slot2(s.b, o1), B2 = slot2(s.b, o2);  // conic c, skip threshold
+            const float p1 = power_of(A1, B1), p2 = power_of(A2, B2);
+            const uint64_t act1 = ballot(p1 <= 0.0f) & ballot(!(p1 < B1.y)) & live;
+            const uint64_t can2 = ballot(p2 <= 0.0f) & ballot(!(p2 < B2.y));
+            if (COUNT) evals += (lane_on(live) && o1 != kSentinel) ? 1u : 0u;
+            if (act1 != 0) blend_one(o1, p1, act1);
+            if (COUNT) evals += (lane_on(live) && o2 != kSentinel) ? 1u : 0u;
+            const uint64_t act2 = can2 & live;
+            if (act2 != 0) blend_one(o2, p2, act2);
             if (live == 0) break;
         }
     }
