@@ -283,14 +283,15 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
         const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, splats);
         if (live == 0) continue;
-        for (uint32_t i = 0; i < walk; ++i) {
-            const uint32_t off = list[i];
+        // two list entries per step, as in blend_tile
+        const auto power_of = [&](uint32_t off) {
             const float4 A = slot4(s.a, off);
             const float2 B = slot2(s.b, off);
             const float dx = A.x - fx, dy = A.y - fy;
             const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & live;
-            if (act == 0) continue;
+            return make_float2(power, B.y);
+        };
+        const auto rerun_one = [&](uint32_t off, float power, uint64_t act) {
             float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;
             const float tn = T * (1.0f - alpha);
@@ -301,6 +302,15 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                 last = (int32_t)(base + off / kSlotBytes);
             }
             live &= ~(ok & low);
+        };
+        for (uint32_t i = 0; i < walk; i += 2) {
+            const uint32_t o1 = list[i], o2 = list[i + 1];
+            const float2 q1 = power_of(o1), q2 = power_of(o2);  // power, skip threshold
+            const uint64_t act1 = ballot(q1.x <= 0.0f) & ballot(!(q1.x < q1.y)) & live;
+            const uint64_t can2 = ballot(q2.x <= 0.0f) & ballot(!(q2.x < q2.y));
+            if (act1 != 0) rerun_one(o1, q1.x, act1);
+            const uint64_t act2 = can2 & live;
+            if (act2 != 0) rerun_one(o2, q2.x, act2);
             if (live == 0) break;
         }
     }
