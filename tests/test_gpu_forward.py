@@ -112,6 +112,34 @@ def test_edge_scenes(eng, orc):
     compare_frame(eng, orc, g, cam, 3, "opacity-extremes-and-duplicates")
 
 
+def test_nearly_equal_depths_every_sort_shape(eng, orc):
+    """Keys that agree in all but their lowest bits: view depths 3 + k ulp(3) (k < 256) put
+    thousands of pairs with nearly equal keys in the centre tiles and a few colliding ones in
+    every other tile.  Every sort shape (8- and 9-bit digits, every block shape) must give the
+    oracle's stable order (tile, depth, emission)."""
+    from torpedo_amd.engine import gaussian_point
+    cam = c1_camera(orc, 160, 96)  # view = [I | (0, 0, 3)]: view z = world z + 3, exactly
+    rng = np.random.default_rng(11)
+    ulp = float(np.spacing(np.float32(3.0)))
+    sh = np.full(48, 0.2, np.float32)
+    recs = []
+    for _ in range(3000):  # around the view axis: thousands of pairs in the centre tiles
+        x, y = rng.normal(0.0, 0.05, 2)
+        recs.append(gaussian_point([x, y, float(rng.integers(0, 256)) * ulp], 0.3, (0, 0, 0, 1),
+                                   (0.01, 0.01, 0.01), sh))
+    for _ in range(600):  # scattered: a few pairs per tile with colliding depths
+        x, y = rng.uniform(-0.8, 0.8, 2)
+        recs.append(gaussian_point([x, y * 0.6, float(rng.integers(0, 6)) * ulp], 0.3, (0, 0, 0, 1),
+                                   (0.01, 0.01, 0.01), sh))
+    g = np.stack(recs).astype(np.float32)
+    try:
+        for shape in range(8):
+            eng.setSortShape(shape)
+            compare_frame(eng, orc, g, cam, 3, f"equal-depth-runs shape {shape}")
+    finally:
+        eng.setSortShape(0)
+
+
 def test_buffer_growth_replay(eng, orc):
     cam = c1_camera(orc, 320, 240)
     compare_frame(eng, orc, small_scene(500, seed=8), cam, 3, "growth-small")
