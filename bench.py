@@ -34,6 +34,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "views/s batched vs plain; fwd+bwd ms @3M Gaussians 1080p; sort % HBM peak"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per pair per k_onesweep launch on this workload (8-bit digits, 512 x 12 shape), from two
+# rocprofv3 --pmc runs: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
+ONESWEEP_TRAFFIC_B_PER_PAIR = 26.78
+ONESWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4: "
+                           "785.2 MB read + 775.7 MB written per launch at 58,288,090 pairs "
+                           "(profiles/batch_frame-multiview32-clusters1M.md)")
 VALU_PEAK_TFLOPS = 157.3
 
 
@@ -61,6 +67,18 @@ def multiview_views(scene_id=0):
     return out
 
 
+# Radix-pass block shapes in the library's table order: (digit bits, threads, keys per thread).
+SORT_SHAPES = [(8, 256, 16), (8, 256, 8), (8, 256, 12), (8, 512, 8), (8, 512, 12), (8, 256, 20), (9, 512, 12),
+               (9, 512, 8)]
+
+
+def shape_desc(i):
+    if i is None or i < 0 or i >= len(SORT_SHAPES):
+        return None
+    b, t, k = SORT_SHAPES[i]
+    return {"index": i, "digit_bits": b, "threads": t, "keys_per_thread": k}
+
+
 def onesweep_roofline(P, sort_passes_ms, passes):
     t_launch = sort_passes_ms / 1000.0 / max(passes, 1)
     achieved = 24.0 * P / t_launch / 1e9 if t_launch > 0 else 0.0
@@ -81,6 +99,12 @@ def time_multiview(eng, g, steps, warmup, tune=True):
             eng.waitIdle()
         return time.perf_counter() - t0
 
+    plain_tuned = None
+    if tune:  # the plain frames get their best radix shape too, tuned on the largest view
+        (_, v, pv, fo, w, h) = max(views, key=lambda x: x[4] * x[5])
+        eng.setCamera(v, pv, fo, w, h)
+        eng.render()
+        plain_tuned = eng.autotune()
     for _ in range(max(warmup, 1)):
         plain_pass()
     t_plain = med([plain_pass() for _ in range(max(3, steps // 4))])
@@ -126,7 +150,7 @@ def time_multiview(eng, g, steps, warmup, tune=True):
             "batched_over_plain": round(t_plain / t_batch, 3),
             "batched_ms_per_frame_wall": round(t_batch * 1000.0, 4), "batched_ms_gpu": round(med(gpu), 4),
             "plain_ms_32_views_wall": round(t_plain * 1000.0, 4), "pairs": P, "plain_pairs_sum": plain_pairs,
-            "eager_stage_ms": stages, "autotune": tuned,
+            "eager_stage_ms": stages, "autotune": tuned, "autotune_plain": plain_tuned,
             "resolutions": [[v[4], v[5]] for v in views]}
 
 
@@ -340,7 +364,8 @@ def main():
     es = mv["eager_stage_ms"]
     ach, t_launch = onesweep_roofline(mv["pairs"], es["sort_passes"], es["radix_passes"])
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_onesweep",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": round(ONESWEEP_TRAFFIC_B_PER_PAIR * mv["pairs"]),
+                "traffic_source": ONESWEEP_TRAFFIC_SOURCE, "kernel": "k_onesweep",
                 "pairs": mv["pairs"], "passes": es["radix_passes"], "avg_launch_us": round(t_launch * 1e6, 2),
                 "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
     ach2, t2 = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
@@ -376,7 +401,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 scenes and camera rigs, no dataset)",
         "config": {"workload": "multiview-32-clusters-1M", "gaussians": 1_000_000, "views": 32,
-                   "radius": 8.0, "sh_degree": 3, "tile": 16, "sort_digit_bits": 8, "pairs": mv["pairs"],
+                   "radius": 8.0, "sh_degree": 3, "tile": 16, "pairs": mv["pairs"],
+                   "sort_shape_batch": shape_desc((mv.get("autotune") or {}).get("best_shape", args.sort_shape)),
+                   "sort_shape_plain": shape_desc((mv.get("autotune_plain") or {}).get("best_shape", args.sort_shape)),
                    "parallelism": "single GPU", "graph": "one recorded HIP graph per frame"},
         "multiview_32": mv,
         "plain_1080p": {"ms_per_frame": round(ms_plain, 4), "views_per_s": round(1000.0 / ms_plain, 2), "pairs": P,
