@@ -64,8 +64,9 @@ struct BatchState {
     uint32_t* d_vals_alt = nullptr; size_t cap_vals_alt = 0;
     uint32_t cap = 0;  // pair capacity used by the recorded frame
     SortWorkspace ws;
-    const uint64_t* sorted_keys = nullptr;
+    const uint64_t* sorted_keys = nullptr;   // 64-bit keys, or 32-bit ones when key_drop != kKeysWide
     const uint32_t* sorted_vals = nullptr;
+    uint32_t key_drop = kKeysWide;
 
     // backward
     float* d_partials = nullptr; size_t cap_partials = 0;
@@ -369,12 +370,12 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     b->passes = sort_shape_passes(b->ws.shape, key_bits);
     const bool in_alt = launch_sort_pairs(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
                                           &b->d_counters[kCtrPairs], b->cap, key_bits, s, events ? b->ev[3] : nullptr,
-                                          31);
+                                          31, &b->key_drop);
     b->sorted_keys = in_alt ? b->d_keys_alt : b->d_keys;
     b->sorted_vals = in_alt ? b->d_vals_alt : b->d_vals;
     if (events) hipEventRecord(b->ev[4], s);
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc, b->nblocks,
-                  &b->d_counters[kCtrScanTicket], s, b->cap);
+                  &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, 31);
     if (events) hipEventRecord(b->ev[5], s);
     launch_batch_blend(blend_args(b), s);
     if (events) hipEventRecord(b->ev[6], s);
@@ -775,8 +776,18 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
     if (!keys && !vals) return TPD_OK;
     if (capacity < b->last_pairs) return fail(TPD_ERR_INVALID, "tpd_batch_read_pairs: capacity too small");
     hipError_t e = hipSuccess;
-    if (keys && b->last_pairs)
-        e = hipMemcpy(keys, b->sorted_keys, sizeof(uint64_t) * b->last_pairs, hipMemcpyDeviceToHost);
+    if (keys && b->last_pairs) {
+        const uint64_t* k = b->sorted_keys;
+        if (b->key_drop != kKeysWide) {
+            // the sort left 32-bit keys: rebuild the 64-bit ones in the other (free) key buffer
+            uint64_t* full = b->sorted_keys == b->d_keys ? b->d_keys_alt : b->d_keys;
+            launch_full_keys(reinterpret_cast<const uint32_t*>(b->sorted_keys), b->key_drop, 31, b->sorted_vals,
+                             b->d_splats, b->last_pairs, full, c->stream);
+            e = hipStreamSynchronize(c->stream);
+            k = full;
+        }
+        if (e == hipSuccess) e = hipMemcpy(keys, k, sizeof(uint64_t) * b->last_pairs, hipMemcpyDeviceToHost);
+    }
     if (e == hipSuccess && vals && b->last_pairs)
         e = hipMemcpy(vals, b->sorted_vals, sizeof(uint32_t) * b->last_pairs, hipMemcpyDeviceToHost);
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_read_pairs");

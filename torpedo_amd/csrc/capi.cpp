@@ -362,6 +362,13 @@ int tpd_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t capacity
     if (unsorted && !c->keep_unsorted) return fail(TPD_ERR_STATE, "tpd_read_pairs: keep_unsorted was not set");
     const uint64_t* k = unsorted ? c->d_keys_dbg : c->sorted_keys;
     const uint32_t* v = unsorted ? c->d_vals_dbg : c->sorted_vals;
+    if (!unsorted && c->key_drop != kKeysWide && c->last_pairs) {
+        // the sort left 32-bit keys: rebuild the 64-bit ones in the other (free) key buffer
+        uint64_t* full = c->sorted_keys == c->d_keys ? c->d_keys_alt : c->d_keys;
+        launch_full_keys(reinterpret_cast<const uint32_t*>(c->sorted_keys), c->key_drop, 31, c->sorted_vals,
+                         c->d_splats, c->last_pairs, full, c->stream);
+        k = full;
+    }
     int rc = copy_back(c, keys, k, sizeof(uint64_t) * (size_t)c->last_pairs);
     if (rc == TPD_OK) rc = copy_back(c, vals, v, sizeof(uint32_t) * (size_t)c->last_pairs);
     return rc;

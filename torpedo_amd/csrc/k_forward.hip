@@ -114,20 +114,27 @@ void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s) {
 // range.slang:16-35.  Empty tiles keep the (0, 0) written by the preprocess kernel (the
 // reference clears with fillBuffer, GaussianEngine.cpp:843-848).  The kernel also re-zeroes the
 // scan descriptors and the scan ticket for the next frame.
-__global__ void k_ranges(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ counters, uint32_t cap,
-                         uint2* __restrict__ ranges, uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket) {
+__global__ void k_ranges(const void* keys, const uint32_t* __restrict__ counters, uint32_t cap,
+                         uint2* __restrict__ ranges, uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket,
+                         uint32_t key_drop, uint32_t lo_bits) {
     const uint32_t P = min(counters[kCtrPairs], cap);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     for (uint32_t d = gid; d < n_scan_desc; d += nthreads) scan_desc[d] = 0;
     if (gid == 0) *ticket = 0;
+    const bool wide = key_drop == kKeysWide;
+    const uint32_t tshift = wide ? 32u : lo_bits - key_drop;
+    const auto tile_of = [&](uint32_t i) -> uint32_t {
+        return wide ? (uint32_t)(static_cast<const uint64_t*>(keys)[i] >> 32)
+                    : static_cast<const uint32_t*>(keys)[i] >> tshift;
+    };
     uint32_t* r = reinterpret_cast<uint32_t*>(ranges);
     for (uint32_t idx = gid; idx < P; idx += nthreads) {
-        const uint32_t cur = (uint32_t)(keys[idx] >> 32);
+        const uint32_t cur = tile_of(idx);
         if (idx == 0) {
             r[2 * cur] = 0;
         } else {
-            const uint32_t prev = (uint32_t)(keys[idx - 1] >> 32);
+            const uint32_t prev = tile_of(idx - 1);
             if (cur != prev) {
                 r[2 * prev + 1] = idx;
                 r[2 * cur] = idx;
@@ -137,13 +144,29 @@ __global__ void k_ranges(const uint64_t* __restrict__ keys, const uint32_t* __re
     }
 }
 
-void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
-                   uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s, uint32_t cap) {
+void launch_ranges(const void* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
+                   uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s, uint32_t cap,
+                   uint32_t key_drop, uint32_t lo_bits) {
     uint32_t work = P_hint > ntiles ? P_hint : ntiles;
     work = work > n_scan_desc ? work : n_scan_desc;
     uint32_t nb = (work + 255) / 256;
     nb = nb < 1 ? 1 : (nb > 8192 ? 8192 : nb);
-    hipLaunchKernelGGL(k_ranges, dim3(nb), dim3(256), 0, s, keys, counters, cap, ranges, scan_desc, n_scan_desc, ticket);
+    hipLaunchKernelGGL(k_ranges, dim3(nb), dim3(256), 0, s, keys, counters, cap, ranges, scan_desc, n_scan_desc, ticket,
+                       key_drop, lo_bits);
+}
+
+__global__ void k_full_keys(const uint32_t* __restrict__ keys32, uint32_t tshift, const uint32_t* __restrict__ vals,
+                            const Splat* __restrict__ splats, uint32_t n, uint64_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = ((uint64_t)(keys32[i] >> tshift) << 32) | as_uint(splats[vals[i]].xyzr.z);
+}
+
+void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, const uint32_t* vals,
+                      const Splat* splats, uint32_t n, uint64_t* out, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t nb = (n + 255) / 256;
+    nb = nb > 8192 ? 8192 : nb;
+    hipLaunchKernelGGL(k_full_keys, dim3(nb), dim3(256), 0, s, keys32, lo_bits - drop, vals, splats, n, out);
 }
 
 // ------------------------------------------------------------------ blend

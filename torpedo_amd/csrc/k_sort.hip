@@ -129,23 +129,29 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
 
 // ------------------------------------------------------------------ one pass
 
+// Key widths of a pass: kWide reads and writes 64-bit keys; kNarrow reads 64-bit keys and writes
+// the 32-bit key (compacted key >> drop), possible once the passes before it have consumed the bits
+// below `drop` and the rest fit in 32 bits; kNarrowed reads and writes such 32-bit keys.
+enum : uint32_t { kWide = 0, kNarrow = 1, kNarrowed = 2 };
+
 struct OnesweepArgs {
-    const uint64_t* keys_in;
+    const void* keys_in;
     const uint32_t* vals_in;
-    uint64_t* keys_out;
+    void* keys_out;
     uint32_t* vals_out;
     const uint32_t* count_ptr;
     uint32_t n_imm;
     uint32_t shift;
     uint32_t lo_bits;          // bits of the key's low word (32, or 31 for positive depths)
+    uint32_t drop;             // kNarrow / kNarrowed: low bits of the compacted key left out of the 32-bit key
     const uint32_t* offsets;   // 2^BITS global exclusive offsets of this pass's digits
     uint32_t* desc;            // this pass's look-back region (parts x 2^BITS), zero on entry
     uint32_t* desc_next;       // the other region, cleared here for the next pass
     uint32_t* ticket;
 };
 
-// LDS of one partition of BLOCK x ITEMS keys with 2^BITS digits.
-template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS>
+// LDS of one partition of BLOCK x ITEMS keys (of type KT) with 2^BITS digits.
+template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS, typename KT>
 struct OnesweepLds {
     static constexpr uint32_t R = 1u << BITS;
     static constexpr uint32_t kWaves = BLOCK / kWave;
@@ -156,7 +162,7 @@ struct OnesweepLds {
     uint32_t adj[R];
     uint32_t wtot[8];
     uint32_t part;
-    uint64_t keys[kTileKeys];   // the partition reordered by digit (then its values)
+    KT keys[kTileKeys];         // the partition reordered by digit (then its values)
     Digit dig[kTileKeys];
 };
 
@@ -198,9 +204,10 @@ __device__ __forceinline__ uint32_t onesweep_lookback(const uint32_t* desc, uint
 
 // The digits are owned by DTH threads, each taking DPT consecutive digits for the per-digit
 // bookkeeping and the look-back.
-template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS>
+template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS, uint32_t MODE>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
-    using Lds = OnesweepLds<BITS, BLOCK, ITEMS>;
+    using KT = typename std::conditional<MODE == kWide, uint64_t, uint32_t>::type;  // key type after the load
+    using Lds = OnesweepLds<BITS, BLOCK, ITEMS, KT>;
     constexpr uint32_t R = Lds::R;
     constexpr uint32_t kWaves = Lds::kWaves;
     constexpr uint32_t kTileKeys = Lds::kTileKeys;
@@ -227,18 +234,24 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const uint32_t base = part * kTileKeys;
     const uint32_t wbase = base + wave * (ITEMS * kWave) + lane;
     const bool full = base + kTileKeys <= n;
-    uint64_t k[ITEMS];
+    const auto load_key = [&](uint32_t idx) -> KT {
+        if constexpr (MODE == kWide) return static_cast<const uint64_t*>(a.keys_in)[idx];
+        if constexpr (MODE == kNarrow)
+            return (uint32_t)(compact_key(static_cast<const uint64_t*>(a.keys_in)[idx], a.lo_bits) >> a.drop);
+        return static_cast<const uint32_t*>(a.keys_in)[idx];
+    };
+    KT k[ITEMS];
     uint32_t v[ITEMS];
     if (full) {
 #pragma unroll
-        for (uint32_t j = 0; j < ITEMS; ++j) k[j] = a.keys_in[wbase + j * kWave];
+        for (uint32_t j = 0; j < ITEMS; ++j) k[j] = load_key(wbase + j * kWave);
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) v[j] = a.vals_in[wbase + j * kWave];
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
             const uint32_t idx = wbase + j * kWave;
-            k[j] = idx < n ? a.keys_in[idx] : ~0ull;
+            k[j] = idx < n ? load_key(idx) : ~KT(0);
         }
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
@@ -252,7 +265,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const uint64_t lt = lanemask_lt();
     uint32_t rank[ITEMS], dig[ITEMS];
 #pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits), a.shift);
+    for (uint32_t j = 0; j < ITEMS; ++j) {
+        if constexpr (MODE == kWide) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits), a.shift);
+        else dig[j] = (k[j] >> (a.shift - a.drop)) & ((1u << BITS) - 1u);
+    }
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = dig[j];
@@ -334,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll 4
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) a.keys_out[L.adj[L.dig[i]] + i] = L.keys[i];
+        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[L.adj[L.dig[i]] + i] = L.keys[i];
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
@@ -349,15 +365,18 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 }
 
 // Shapes the tuner chooses from: {digit bits, threads, keys per thread}.
+// kernel[mode] for the key widths kWide, kNarrow, kNarrowed.
 struct SortShape {
     uint32_t bits, block, items;
-    void (*kernel)(const OnesweepArgs);
+    void (*kernel[3])(const OnesweepArgs);
 };
+#define TPD_SHAPE(B, T, I) \
+    { B, T, I, { k_onesweep<B, T, I, kWide>, k_onesweep<B, T, I, kNarrow>, k_onesweep<B, T, I, kNarrowed> } }
 static const SortShape kShapes[] = {
-    {8, 256, 16, k_onesweep<8, 256, 16>}, {8, 256, 8, k_onesweep<8, 256, 8>},   {8, 256, 12, k_onesweep<8, 256, 12>},
-    {8, 512, 8, k_onesweep<8, 512, 8>},   {8, 512, 12, k_onesweep<8, 512, 12>}, {8, 256, 20, k_onesweep<8, 256, 20>},
-    {9, 512, 12, k_onesweep<9, 512, 12>}, {9, 512, 8, k_onesweep<9, 512, 8>},
+    TPD_SHAPE(8, 256, 16), TPD_SHAPE(8, 256, 8),  TPD_SHAPE(8, 256, 12), TPD_SHAPE(8, 512, 8),
+    TPD_SHAPE(8, 512, 12), TPD_SHAPE(8, 256, 20), TPD_SHAPE(9, 512, 12), TPD_SHAPE(9, 512, 8),
 };
+#undef TPD_SHAPE
 constexpr uint32_t kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 uint32_t sort_num_shapes() { return kNumShapes; }
@@ -378,7 +397,7 @@ static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const uint6
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist, uint32_t lo_bits) {
+                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -391,6 +410,17 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
     else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
+    // With narrow_drop, the first pass whose digit starts at or above key_bits - 32 writes 32-bit
+    // keys (the bits from its digit up) and the later passes move 8 instead of 12 bytes per pair.
+    uint32_t narrow_pass = passes;
+    if (narrow_drop) {
+        for (uint32_t p = 0; p < passes; ++p)
+            if (key_bits <= p * sh.bits + 32) {
+                narrow_pass = p;
+                break;
+            }
+        *narrow_drop = narrow_pass < passes ? narrow_pass * sh.bits : kKeysWide;
+    }
     const uint64_t* kin = keys;
     const uint32_t* vin = vals;
     uint64_t* kout = keys_alt;
@@ -401,11 +431,13 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.count_ptr = count_ptr; o.n_imm = n_max;
         o.shift = p * sh.bits;
         o.lo_bits = lo_bits;
+        o.drop = narrow_pass < passes ? narrow_pass * sh.bits : 0u;
+        const uint32_t mode = p < narrow_pass ? kWide : (p == narrow_pass ? kNarrow : kNarrowed);
         o.offsets = ws.offsets + p * R;
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
         o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * R;
         o.ticket = ws.tickets + p;
-        hipLaunchKernelGGL(sh.kernel, dim3(parts), dim3(sh.block), 0, s, o);
+        hipLaunchKernelGGL(sh.kernel[mode], dim3(parts), dim3(sh.block), 0, s, o);
         const uint64_t* tk = kout; kout = const_cast<uint64_t*>(kin); kin = tk;
         const uint32_t* tv = vout; vout = const_cast<uint32_t*>(vin); vin = tv;
     }

@@ -36,6 +36,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
         c->last_pairs = 0;
         c->sorted_keys = c->d_keys;
         c->sorted_vals = c->d_vals;
+        c->key_drop = kKeysWide;
         return TPD_OK;
     }
 
@@ -98,9 +99,10 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     if (c->profile) hipEventRecord(c->ev[2], s);
     if (P > 0)
         in_alt = launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, P, key_bits, s,
-                                   c->profile ? c->ev[3] : nullptr, 31);
-    else if (c->profile)
-        hipEventRecord(c->ev[3], s);
+                                   c->profile ? c->ev[3] : nullptr, 31, &c->key_drop);
+    else
+        c->key_drop = kKeysWide;
+    if (P == 0 && c->profile) hipEventRecord(c->ev[3], s);
     if (c->profile) hipEventRecord(c->ev[4], s);
     c->sorted_keys = in_alt ? c->d_keys_alt : c->d_keys;
     c->sorted_vals = in_alt ? c->d_vals_alt : c->d_vals;
@@ -112,7 +114,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
             return hip_fail(e, "clear evaluation counter");
     }
     launch_ranges(c->sorted_keys, c->d_counters, P, ntiles, c->d_ranges, c->d_scan_desc, nd,
-                  &c->d_counters[kCtrScanTicket], s);
+                  &c->d_counters[kCtrScanTicket], s, 0xFFFFFFFFu, c->key_drop, 31);
     if (c->profile) hipEventRecord(c->ev[5], s);
     BlendArgs b{};
     b.ranges = c->d_ranges;

@@ -49,7 +49,8 @@ struct tpd_ctx {
     uint64_t* d_keys_dbg = nullptr; size_t cap_keys_dbg = 0;
     uint32_t* d_vals_dbg = nullptr; size_t cap_vals_dbg = 0;
     tpd::SortWorkspace ws;
-    const uint64_t* sorted_keys = nullptr;
+    const uint64_t* sorted_keys = nullptr;   // 64-bit keys, or 32-bit ones when key_drop != kKeysWide
+    uint32_t key_drop = tpd::kKeysWide;
     const uint32_t* sorted_vals = nullptr;
 
     // image

@@ -76,11 +76,18 @@ void launch_preprocess_bwd(const PreBackwardArgs& a, hipStream_t s);
 
 void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
 void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
+// Sorted keys are 64-bit (tile << 32) | depth, or, after a sort that narrowed them, 32-bit
+// (compacted key (tile << lo_bits) | depth) >> drop; key_drop = kKeysWide marks the first form.
+constexpr uint32_t kKeysWide = 0xFFFFFFFFu;
 // Tile ranges of the sorted keys; P = min(counters[kCtrPairs], cap).  Also re-zeroes the scan
 // descriptors and the scan ticket for the next frame.
-void launch_ranges(const uint64_t* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
+void launch_ranges(const void* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s,
-                   uint32_t cap = 0xFFFFFFFFu);
+                   uint32_t cap = 0xFFFFFFFFu, uint32_t key_drop = kKeysWide, uint32_t lo_bits = 31);
+// The 64-bit keys (tile << 32) | floatbits(view depth) of n sorted 32-bit keys, the tile from the
+// key and the depth from the splat of each value (for readers of the sorted pairs).
+void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, const uint32_t* vals,
+                      const Splat* splats, uint32_t n, uint64_t* out, hipStream_t s);
 void launch_blend(const BlendArgs& a, hipStream_t s);
 
 // ------------------------------------------------------------------ batched rendering
@@ -214,8 +221,11 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits);  // passes for ke
 // after_hist (optional) is recorded between the histogram kernels and the first onesweep pass.
 // lo_bits < 32 sorts the compacted key (high word << lo_bits) | low word, valid when every key's
 // low word fits in lo_bits bits; key_bits then counts the compacted key's bits.
+// narrow_drop (optional) lets the later passes move 32-bit keys: *narrow_drop receives the number
+// of low compacted-key bits the sorted 32-bit keys leave out, or kKeysWide if the keys stayed
+// 64-bit (see launch_ranges).
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32);
+                       hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr);
 
 }  // namespace tpd

@@ -74,8 +74,9 @@ int tpd_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shap
         for (int rep = 0; rep < 4 && rc == TPD_OK; ++rep) {
             if ((rc = rerun_duplicate(c, s)) != TPD_OK) break;
             hipEventRecord(e0, s);
+            uint32_t drop = kKeysWide;  // with the narrowed passes, as render_forward sorts
             launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, c->last_pairs,
-                              key_bits, s, nullptr, 31);
+                              key_bits, s, nullptr, 31, &drop);
             hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) {
                 rc = fail(TPD_ERR_HIP, "tpd_autotune: sort failed");
