@@ -195,7 +195,7 @@ constexpr uint32_t kMaxRadixBits = 9;                      // digit widths 8 and
 constexpr uint32_t kMaxRadix = 1u << kMaxRadixBits;
 constexpr uint32_t kSortMinTile = 2048;                    // smallest partition of any block shape
 constexpr uint32_t kMaxPasses = 8;
-constexpr uint32_t kHistBlocks = 512;                      // blocks of the digit-histogram kernel
+constexpr uint32_t kHistBlocks = 1024;                      // blocks of the digit-histogram kernel
 constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back values
 
 struct SortWorkspace {
