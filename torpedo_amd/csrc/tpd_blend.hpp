@@ -76,10 +76,31 @@ __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uin
 // alpha >= 1/255 at some pixel centre of the sub-block (splat_sub_block_mask).  Returns the number
 // of list entries this wave walks: the longest of its four lists (the shorter ones are padded
 // with the sentinel).  Ends with a barrier.
+// This thread's splat of a staging round: whether the slot holds one, its index and its record.
+struct StagedSplat {
+    bool valid;
+    uint32_t g;
+    Splat sp;
+};
+
+// Both loads are unconditional (slot past the end: value 0 and record 0, which every non-empty
+// buffer has), so that a load issued a round ahead stays in flight in its own registers.
+__device__ __forceinline__ uint32_t load_value(const uint32_t* __restrict__ vals, uint32_t k, uint32_t end) {
+    return vals[k < end ? k : 0u];
+}
+
+__device__ __forceinline__ StagedSplat load_staged(const Splat* __restrict__ splats, uint32_t g, uint32_t k,
+                                                   uint32_t end) {
+    StagedSplat t;
+    t.valid = k < end;
+    t.g = g;
+    t.sp = splats[t.valid ? g : 0u];
+    return t;
+}
+
 template <bool STORE_G>
-__device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
-                                                uint32_t end, const uint32_t* __restrict__ vals,
-                                                const Splat* __restrict__ splats) {
+__device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_g, const TileView& tv,
+                                                       const StagedSplat& in) {
     const uint32_t tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = __lane_id();
     {
         constexpr uint32_t s2 = kSentinel | (kSentinel << 16);
@@ -93,11 +114,10 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
             s.c[kTilePixels] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
-    const uint32_t k = base + tid;
     uint32_t mask = 0;
-    if (k < end) {
-        const uint32_t g = vals[k];
-        const Splat& sp = splats[g];
+    if (in.valid) {
+        const uint32_t g = in.g;
+        const Splat& sp = in.sp;
         const float4 p = sp.xyzr;
         const float4 c = sp.conop;
         s.a[tid] = make_float4(p.x, p.y, c.x, c.y);
@@ -147,6 +167,14 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
     return walk;
 }
 
+template <bool STORE_G>
+__device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
+                                                uint32_t end, const uint32_t* __restrict__ vals,
+                                                const Splat* __restrict__ splats) {
+    const uint32_t k = base + threadIdx.x;
+    return blend_stage_loaded<STORE_G>(s, s_g, tv, load_staged(splats, load_value(vals, k, end), k, end));
+}
+
 // Forward composite of one tile.  Per pixel the operations and their order are the reference's;
 // the shortcuts around them change no pixel:
 //   - the sub-block lists of blend_stage drop only splats no pixel of the sub-block can take;
@@ -173,9 +201,15 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     uint64_t live = ballot(inside);
     uint32_t evals = 0;
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
+    // The global loads of staging are issued a round ahead: the record of round r + 1 and the
+    // value of round r + 2 are in flight while round r blends.
+    StagedSplat next = load_staged(splats, load_value(vals, range.x + tid, range.y), range.x + tid, range.y);
+    uint32_t g_after = load_value(vals, range.x + kTilePixels + tid, range.y);
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
         if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t walk = blend_stage<false>(s, nullptr, tv, base, range.y, vals, splats);
+        const uint32_t walk = blend_stage_loaded<false>(s, nullptr, tv, next);
+        next = load_staged(splats, g_after, base + kTilePixels + tid, range.y);
+        g_after = load_value(vals, base + 2 * kTilePixels + tid, range.y);
         if (live == 0) continue;
         // Two list entries per step: both splats' powers are formed before the first one updates
         // the pixel (power does not depend on T), then the two updates run in list order.  An odd
