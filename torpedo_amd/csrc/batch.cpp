@@ -54,6 +54,7 @@ struct BatchState {
     uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
     uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
+    uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
     float* d_image = nullptr; size_t cap_image = 0;
     uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
 
@@ -115,7 +116,7 @@ void batch_release(tpd_ctx* c) {
     release(b->d_counters);
     if (b->h_counters) (void)hipHostFree(b->h_counters);
     release(b->d_splats); release(b->d_tiles); release(b->d_offsets);
-    release(b->d_ranges); release(b->d_image); release(b->d_rgba8);
+    release(b->d_ranges); release(b->d_tile_order); release(b->d_image); release(b->d_rgba8);
     release(b->d_keys); release(b->d_vals); release(b->d_keys_alt); release(b->d_vals_alt);
     release(b->ws.desc); release(b->ws.partials); release(b->ws.offsets); release(b->ws.tickets);
     release(b->d_partials); release(b->d_grads);
@@ -227,6 +228,7 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_splats, b->cap_splats, it)) != hipSuccess || (e = grow(b->d_tiles, b->cap_tiles, it)) != hipSuccess ||
         (e = grow(b->d_offsets, b->cap_offsets, it)) != hipSuccess ||
         (e = grow(b->d_ranges, b->cap_ranges, std::max<size_t>(tiles, 1))) != hipSuccess ||
+        (e = grow(b->d_tile_order, b->cap_tile_order, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_image, b->cap_image, std::max<size_t>(3 * pix, 1))) != hipSuccess ||
         (e = grow(b->d_rgba8, b->cap_rgba8, std::max<size_t>(4 * pix, 1))) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc");
@@ -344,6 +346,7 @@ static BatchBlendArgs blend_args(BatchState* b) {
     BatchBlendArgs a{};
     a.views = b->d_views;
     a.tile_view = b->d_tile_view;
+    a.tile_order = b->d_tile_order;
     a.ranges = b->d_ranges;
     a.vals = b->sorted_vals;
     a.splats = b->d_splats;
@@ -376,6 +379,7 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     if (events) hipEventRecord(b->ev[4], s);
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc, b->nblocks,
                   &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, 31);
+    launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);
     launch_batch_blend(blend_args(b), s);
     if (events) hipEventRecord(b->ev[6], s);

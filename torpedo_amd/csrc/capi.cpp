@@ -56,7 +56,8 @@ int ensure_image_buffers(tpd_ctx* c) {
     const size_t ntiles = (size_t)gx * gy;
     const size_t npix = (size_t)c->cam.width * c->cam.height;
     hipError_t e;
-    if ((e = grow(c->d_ranges, c->cap_ranges, std::max<size_t>(ntiles, 1))) != hipSuccess)
+    if ((e = grow(c->d_ranges, c->cap_ranges, std::max<size_t>(ntiles, 1))) != hipSuccess ||
+        (e = grow(c->d_tile_order, c->cap_tile_order, std::max<size_t>(ntiles, 1))) != hipSuccess)
         return hip_fail(e, "alloc ranges");
     if ((e = grow(c->d_image, c->cap_image, std::max<size_t>(3 * npix, 1))) != hipSuccess)
         return hip_fail(e, "alloc image");
@@ -182,6 +183,7 @@ int tpd_destroy(tpd_ctx* c) {
     release(c->ws.offsets);
     release(c->ws.tickets);
     release(c->d_ranges);
+    release(c->d_tile_order);
     release(c->d_image);
     release(c->d_rgba8);
     release(c->d_eval_count);

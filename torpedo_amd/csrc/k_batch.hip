@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArg
                                                              const uint32_t* __restrict__ tile_view) {
     __shared__ BlendLds lds;
     if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
     blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, nullptr);
 }

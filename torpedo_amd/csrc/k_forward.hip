@@ -169,6 +169,70 @@ void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, c
     hipLaunchKernelGGL(k_full_keys, dim3(nb), dim3(256), 0, s, keys32, lo_bits - drop, vals, splats, n, out);
 }
 
+// ------------------------------------------------------------------ tile order
+
+// One block of 1024 threads: count the tiles per class (bit length of the pair count, 0 for an
+// empty tile), give each class its start (heaviest class first), then scatter every tile to its
+// class; within a class the order is the order in which waves claim slots.  Every wave combines
+// its lanes of one class into one LDS atomic.
+constexpr uint32_t kOrderThreads = 1024;
+
+__device__ __forceinline__ uint32_t tile_class(const uint2* __restrict__ ranges, uint32_t t) {
+    const uint2 r = ranges[t];
+    const uint32_t n = r.y > r.x ? r.y - r.x : 0u;
+    return n ? 32u - (uint32_t)__builtin_clz(n) : 0u;  // 0 .. 32
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint2* __restrict__ ranges, uint32_t ntiles,
+                                                              const uint32_t* counters, uint32_t* __restrict__ order) {
+    __shared__ uint32_t slot[33];
+    const uint32_t tid = threadIdx.x, lane = __lane_id();
+    if (counters && counters[kCtrOverflow]) return;
+    if (tid < 33) slot[tid] = 0;
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    // claims count tiles (pass 0) or slots (pass 1) for the lanes of each class, one atomic per class
+    const auto claim = [&](uint32_t cls, bool valid) -> uint32_t {
+        uint64_t todo = ballot(valid);
+        uint32_t mine = 0;
+        while (todo) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+            const uint32_t c = __builtin_amdgcn_readlane(cls, leader);
+            const uint64_t peers = ballot(valid && cls == c) & todo;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&slot[c], (uint32_t)__popcll(peers));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if ((peers >> lane) & 1ull) mine = base + (uint32_t)__popcll(peers & lt);
+            todo &= ~peers;
+        }
+        return mine;
+    };
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
+        const uint32_t t = t0 + tid;
+        claim(t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t pos = 0;
+        for (int c = 32; c >= 0; --c) {
+            const uint32_t n = slot[c];
+            slot[c] = pos;
+            pos += n;
+        }
+    }
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
+        const uint32_t t = t0 + tid;
+        const uint32_t pos = claim(t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
+        if (t < ntiles) order[pos] = t;
+    }
+}
+
+void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* counters, uint32_t* order, hipStream_t s) {
+    if (ntiles == 0) return;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, ranges, ntiles, counters, order);
+}
+
 // ------------------------------------------------------------------ blend
 
 // blend.slang:22-105.  One 16x16 tile per 256-thread block (blend_tile); block indices are
@@ -176,7 +240,7 @@ void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, c
 template <bool COUNT>
 __global__ __launch_bounds__(kTilePixels) void k_blend(const BlendArgs a) {
     __shared__ BlendLds lds;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv{tile % a.gx, tile / a.gx, a.width, a.height, a.gx, 0, 0};
     blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, a.eval_count);
 }

@@ -115,9 +115,11 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     }
     launch_ranges(c->sorted_keys, c->d_counters, P, ntiles, c->d_ranges, c->d_scan_desc, nd,
                   &c->d_counters[kCtrScanTicket], s, 0xFFFFFFFFu, c->key_drop, 31);
+    launch_tile_order(c->d_ranges, ntiles, nullptr, c->d_tile_order, s);
     if (c->profile) hipEventRecord(c->ev[5], s);
     BlendArgs b{};
     b.ranges = c->d_ranges;
+    b.tile_order = c->d_tile_order;
     b.vals = c->sorted_vals;
     b.splats = c->d_splats;
     b.width = W;

@@ -55,6 +55,7 @@ struct tpd_ctx {
 
     // image
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
+    uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
     float* d_image = nullptr; size_t cap_image = 0;
     uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
 

@@ -34,6 +34,7 @@ struct ForwardArgs {
 
 struct BlendArgs {
     const uint2* ranges;
+    const uint32_t* tile_order;  // block -> tile (heaviest first, launch_tile_order), or null
     const uint32_t* vals;
     const Splat* splats;
     uint32_t width, height, gx, ntiles;
@@ -89,6 +90,10 @@ void launch_ranges(const void* keys, const uint32_t* counters, uint32_t P_hint, 
 void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, const uint32_t* vals,
                       const Splat* splats, uint32_t n, uint64_t* out, hipStream_t s);
 void launch_blend(const BlendArgs& a, hipStream_t s);
+// The tiles in launch order for the blend: classes of pair count (bit length) from the heaviest
+// down, empty tiles last, so that the longest tiles start first and the short ones fill the tail.
+// counters (optional): nothing is done after a pair overflow.
+void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* counters, uint32_t* order, hipStream_t s);
 
 // ------------------------------------------------------------------ batched rendering
 //
@@ -146,6 +151,7 @@ struct BatchArgs {
 struct BatchBlendArgs {
     const BatchViewDev* views;
     const uint32_t* tile_view;    // view of each global tile
+    const uint32_t* tile_order;   // block -> tile (heaviest first), or null
     const uint2* ranges;
     const uint32_t* vals;
     const Splat* splats;
