@@ -67,6 +67,7 @@ extern "C" int tpd_backward(tpd_ctx* c, const float* d_dL_dimage, float* d_grad_
     if (c->last_pairs > 0) {
         BackwardArgs a{};
         a.ranges = c->d_ranges;
+        a.tile_order = c->d_tile_order;
         a.vals = c->sorted_vals;
         a.splats = c->d_splats;
         a.width = W;

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlen
                                                                  const uint32_t* __restrict__ tile_view) {
     __shared__ BlendBwdLds lds;
     if (a.counters[kCtrOverflow]) return;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
     blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials);
 }

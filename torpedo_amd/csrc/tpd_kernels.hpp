@@ -45,6 +45,7 @@ struct BlendArgs {
 
 struct BackwardArgs {
     const uint2* ranges;
+    const uint32_t* tile_order;  // the forward frame's block -> tile order, or null
     const uint32_t* vals;
     const Splat* splats;
     uint32_t width, height, gx, gy, ntiles;
