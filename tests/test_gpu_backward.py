@@ -102,9 +102,13 @@ def test_backward_accumulate_and_empty(eng, orc):
     torch.cuda.synchronize()
     eng.backward(up.data_ptr(), *[o.data_ptr() for o in once])
     eng.waitIdle()
-    # float atomics sum in a run-dependent order: equal up to rounding, not bit for bit
+    # LDS float atomics sum each tile's per-pixel terms in a run-dependent order, so two runs
+    # agree up to rounding only, and terms that cancel leave that rounding large beside the
+    # result entry by entry (2^-14 seen on a mean gradient of 4.5).  The test is therefore
+    # norm-wise, to the same 1e-4 bar as the FP64 comparison; ignoring `accumulate` would
+    # leave a relative error of 0.5.
     for a, b in zip(outs, once):
-        assert float((a - 2 * b).abs().max()) <= 1e-5 * float(b.abs().max()) + 1e-12
+        assert torch_ref.relative_error(a.cpu(), (2 * b).cpu()) < TOL
     # all Gaussians behind the camera: zero gradients
     far = small_scene(100, seed=10)
     far[:, 2] -= 10.0

@@ -278,8 +278,17 @@ static int upload_tables(BatchState* b) {
 
 static int ensure_pairs(BatchState* b, size_t pairs) {
     pairs = std::max<size_t>(pairs, 1);
-    if (b->cap_keys >= pairs && b->cap_vals >= pairs && b->cap_keys_alt >= pairs && b->cap_vals_alt >= pairs)
+    if (b->cap_keys >= pairs && b->cap_vals >= pairs && b->cap_keys_alt >= pairs && b->cap_vals_alt >= pairs) {
+        // The buffers fit already (a new layout after a larger one): only the frame's capacity,
+        // which a new layout sets to zero, is restored.
+        const uint32_t cap = (uint32_t)std::min(
+            {b->cap_keys, b->cap_vals, b->cap_keys_alt, b->cap_vals_alt, (size_t)kMaxSortKeys});
+        if (b->cap != cap) {
+            destroy_graphs(b);
+            b->cap = cap;
+        }
         return TPD_OK;
+    }
     const double h = 1.15;
     hipError_t e;
     if ((e = grow(b->d_keys, b->cap_keys, pairs, h)) != hipSuccess ||
