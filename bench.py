@@ -217,6 +217,63 @@ def time_fwd_bwd(eng, steps, warmup):
             "width": W, "height": H, "stage_ms": {k: v for k, v in m.items() if k not in ("radix_passes",)}}
 
 
+def pipeline_cameras(n, w=1920, h=1080):
+    """n cameras on a circle of radius 8 around the clusters-1M scene (one frame each)."""
+    out = []
+    for i in range(n):
+        a = 2.0 * np.pi * i / n
+        v, pv, fo = camera_uniforms([8.0 * np.sin(a), 0.0, -8.0 * np.cos(a)], w, h)
+        out.append((0, v, pv, fo, w, h))
+    return out
+
+
+def time_pipelined(g, frames, warmup):
+    """Frame pipelining: a stream of single-view 1920x1080 frames of the clusters-1M scene, (a) one context,
+    each frame set up, launched and waited for, (b) two contexts on their own streams taking frames
+    in turn, so that frame i + 1 is launched while frame i still runs and the host waits only for
+    frame i - 1 before reusing its context.  Both contexts read one device copy of the records."""
+    from torpedo_amd.engine import GaussianEngine
+    cams = pipeline_cameras(frames + warmup)
+    engs = [GaussianEngine(device=0), GaussianEngine(device=0)]
+    d_rec = engs[0].alloc(g.nbytes)
+    engs[0].upload(d_rec, g)
+    for e in engs:
+        e.clearScenes()
+        e.addDeviceScene(d_rec, g.shape[0], 3)
+        e.setBatchGraph(True)
+        e.setViews([cams[0]])
+        e.rasterBatch()
+        e.syncBatch()
+    # (a) one context, one frame at a time
+    a = engs[0]
+    for c in cams[:warmup]:
+        a.setViews([c]); a.rasterBatch(); a.syncBatch()
+    t0 = time.perf_counter()
+    for c in cams[warmup:]:
+        a.setViews([c]); a.rasterBatch(); a.syncBatch()
+    t_seq = (time.perf_counter() - t0) / frames
+    # (b) two contexts, two frames in flight
+    def run(seq):
+        for i, c in enumerate(seq):
+            e = engs[i % 2]
+            e.setViews([c])  # waits for this context's previous frame (two frames back)
+            e.rasterBatch()
+        for e in engs:
+            e.syncBatch()
+    run(cams[:warmup])
+    t0 = time.perf_counter()
+    run(cams[warmup:])
+    t_pipe = (time.perf_counter() - t0) / frames
+    pairs = engs[0].batchPairCount
+    engs[0].free(d_rec)
+    for e in engs:
+        e.close()
+    return {"frames": frames, "width": 1920, "height": 1080, "pairs_last_frame": pairs,
+            "sequential_ms_per_frame": round(t_seq * 1000.0, 4), "sequential_frames_per_s": round(1.0 / t_seq, 2),
+            "pipelined_ms_per_frame": round(t_pipe * 1000.0, 4), "pipelined_frames_per_s": round(1.0 / t_pipe, 2),
+            "pipelined_over_sequential": round(t_seq / t_pipe, 3)}
+
+
 def time_multiscene(eng, steps, warmup):
     """16 scenes x 4 views, forward + backward of all 64 views in one recorded graph."""
     from torpedo_amd import scenes
@@ -295,7 +352,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
-    ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene"], default=None,
+    ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene", "pipelined"],
+                    default=None,
                     help="run one leg only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -335,6 +393,11 @@ def main():
         return
     if args.only == "multiscene":
         print(json.dumps(time_multiscene(eng, args.steps, args.warmup)), flush=True)
+        eng.close()
+        return
+    if args.only == "pipelined":
+        g = scenes.cluster_scene(1_000_000, seed=2)
+        print(json.dumps(time_pipelined(g, max(args.steps, 8), max(args.warmup, 2))), flush=True)
         eng.close()
         return
 
@@ -378,6 +441,9 @@ def main():
     log("[bench] 16 scenes x 4 views training step")
     ms5 = time_multiscene(eng, max(5, args.steps // 2), 2)
     log(f"[bench] multiscene: {ms5}")
+    log("[bench] frame pipelining, 1920x1080 single-view frames, two contexts")
+    pipe = time_pipelined(g, max(16, args.steps), 4)
+    log(f"[bench] pipelined: {pipe}")
     copy_gbs = eng.measureCopy(1 << 30, 10)
     log(f"[bench] plain copy: {copy_gbs:.0f} GB/s")
     # Blend: ~22 FP32 operations + 1 exp per evaluated pixel-splat pair (device-counted).
@@ -418,6 +484,7 @@ def main():
         "roofline_blend": roofline_blend,
         "fwd_bwd_3M_1080p": fb,
         "multiscene_16x4_step": ms5,
+        "pipelined_1080p": pipe,
         "copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_cpu_baseline:

@@ -142,6 +142,12 @@ def test_batch_camera_update_and_growth(engines):
     for _ in range(3):  # replays give the same frame
         beng.rasterBatch()
     check_batch(beng, plain, [g], [3], near, "near cameras replayed")
+    # a new, smaller layout whose pairs fit the buffers already there: the frame's capacity is
+    # restored from those buffers, not left at the zero a new layout starts from
+    small = [view_tuple(sid, (0.0, 0.3, -2.5), 200, 120)]
+    beng.setViews(small)
+    beng.rasterBatch()
+    check_batch(beng, plain, [g], [3], small, "smaller layout in the grown buffers")
 
 
 def test_batch_entities_with_transforms(engines):
@@ -275,3 +281,47 @@ def test_batch_backward_equals_sum_of_plain(engines, use_step):
         for b in bufs:
             beng.free(b)
     beng.free(d_dl)
+
+
+def test_two_frames_in_flight(engines):
+    """Frame pipelining (SurfaceRenderer.h:66 keeps two frames in flight): two contexts on their own
+    streams take frames in turn, each reading one device copy of the records, and a context is
+    reused only after its frame from two steps back has been read.  Every frame must equal the
+    plain frame of the same camera (FP32 and RGBA8 bit-exact)."""
+    from torpedo_amd.engine import GaussianEngine
+    _, plain = engines
+    g = small_scene(20000, seed=31, log_scale_mean=-3.5)
+    cams = []
+    for i in range(7):
+        a = 2.0 * np.pi * i / 7
+        cams.append(view_tuple(0, [3.0 * np.sin(a), 0.4 * i - 1.2, -3.0 * np.cos(a)], 320 + 16 * i, 200 + 8 * i))
+    engs = [GaussianEngine(), GaussianEngine()]
+    d_rec = engs[0].alloc(g.nbytes)
+    try:
+        engs[0].upload(d_rec, g)
+        for e in engs:
+            e.clearScenes()
+            e.addDeviceScene(d_rec, g.shape[0], 3)
+            e.setBatchGraph(True)
+        got = {}
+        for i, c in enumerate(cams):
+            e = engs[i % 2]
+            if i >= 2:
+                e.syncBatch()
+                got[i - 2] = (e.readBatchImage(0), e.readBatchRGBA8(0))
+            e.setViews([c])
+            e.rasterBatch()
+        for i in (len(cams) - 2, len(cams) - 1):
+            e = engs[i % 2]
+            e.syncBatch()
+            got[i] = (e.readBatchImage(0), e.readBatchRGBA8(0))
+        plain.uploadRecords(g, 3)
+        for i, c in enumerate(cams):
+            _, _, img, rgba = plain_frame(plain, c)
+            say(f"[batch] in-flight frame {i}: {c[4]}x{c[5]}")
+            assert np.array_equal(got[i][0].view(np.uint32), img.view(np.uint32)), i
+            assert np.array_equal(got[i][1], rgba), i
+    finally:
+        engs[0].free(d_rec)
+        for e in engs:
+            e.close()
