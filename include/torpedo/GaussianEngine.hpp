@@ -130,6 +130,12 @@ public:
         check(tpd_read_ranges(_ctx, r.data()), "GaussianEngine::readRanges");
         return r;
     }
+    // draw(SwapImage) (GaussianEngine.h:27, GaussianEngine.cpp:714-762, 865-875): the RGBA8 target
+    // into a device surface of row pitch `pitch` bytes (0: 4 * width), ordered on `stream`.
+    void draw(void* surface, uint64_t pitch = 0, void* stream = nullptr) const {
+        check(tpd_present_rgba8(_ctx, surface, pitch, stream), "GaussianEngine::draw");
+    }
+    void savePNG(const std::string& path) const { check(tpd_save_png(_ctx, path.c_str()), "GaussianEngine::savePNG"); }
     [[nodiscard]] float* deviceImage() const {
         float* p = nullptr;
         check(tpd_device_image(_ctx, &p), "GaussianEngine::deviceImage");
@@ -196,6 +202,9 @@ public:
         std::vector<float> img(3 * (size_t)s.first * s.second);
         check(tpd_batch_read_image_f32(_ctx, view, img.data()), "GaussianEngine::readBatchImage");
         return img;
+    }
+    void saveBatchPNG(uint32_t view, const std::string& path) const {
+        check(tpd_batch_save_png(_ctx, view, path.c_str()), "GaussianEngine::saveBatchPNG");
     }
     [[nodiscard]] uint32_t batchPairCount() const {
         uint32_t p = 0;

@@ -101,6 +101,16 @@ int tpd_device_image(tpd_ctx* ctx, float** d_rgb);
 /* Device-to-device copy of the FP32 image into d_dst (H*W*3 floats), ordered on stream. */
 int tpd_copy_image(tpd_ctx* ctx, float* d_dst, void* stream);
 
+/* ----- presentation (GaussianEngine::draw + recordTargetCopy, volumetric/src/GaussianEngine.cpp:714-762,
+ * 865-875: the R8G8B8A8 target, :318, is copied into the swapchain image after each frame) -----
+ * tpd_present_rgba8 copies the RGBA8 image into a caller's device surface of row pitch dst_pitch
+ * bytes (0 = 4 * width), ordered on stream (NULL: the context's stream).  tpd_save_png writes it
+ * to a PNG file (RGBA, 8 bits per channel) for callers without a window; tpd_write_png encodes a
+ * host RGBA8 array (H*W*4 bytes, rows top to bottom) the same way and needs no GPU. */
+int tpd_present_rgba8(tpd_ctx* ctx, void* d_dst, uint64_t dst_pitch, void* stream);
+int tpd_save_png(tpd_ctx* ctx, const char* path);
+int tpd_write_png(const uint8_t* rgba, uint32_t width, uint32_t height, const char* path);
+
 /* ----- backward (the reference's differentiable rasterizer is an empty stub:
  * differential/include/torpedo/differential/GaussianRasterizer.h:1-8, README.md:8) -----
  * Gradients of the last rendered frame with respect to the uploaded/bound records, expressed in
@@ -214,6 +224,9 @@ int tpd_batch_view_layout(tpd_ctx* ctx, uint32_t view, uint32_t* tile_base, uint
 /* Readbacks of one view's image (H x W x 3 float, H x W x 4 UNORM8). */
 int tpd_batch_read_image_f32(tpd_ctx* ctx, uint32_t view, float* host);
 int tpd_batch_read_image_rgba8(tpd_ctx* ctx, uint32_t view, uint8_t* host);
+/* Presentation of one view's RGBA8 image, as tpd_present_rgba8 / tpd_save_png (waits for the frame). */
+int tpd_batch_present_rgba8(tpd_ctx* ctx, uint32_t view, void* d_dst, uint64_t dst_pitch, void* stream);
+int tpd_batch_save_png(tpd_ctx* ctx, uint32_t view, const char* path);
 /* Device pointer of the concatenated float images (owned by the context) and their total pixels. */
 int tpd_batch_device_images(tpd_ctx* ctx, float** d_images, uint64_t* total_pixels);
 /* The sorted global (key, value) pairs of the last batch frame; *P receives the count. */

@@ -58,6 +58,9 @@ SIGNATURES = {
     "tpd_read_ranges": (_i32, [_vp, _vp]),
     "tpd_device_image": (_i32, [_vp, ctypes.POINTER(_vp)]),
     "tpd_copy_image": (_i32, [_vp, _vp, _vp]),
+    "tpd_present_rgba8": (_i32, [_vp, _vp, ctypes.c_uint64, _vp]),
+    "tpd_save_png": (_i32, [_vp, ctypes.c_char_p]),
+    "tpd_write_png": (_i32, [_vp, _u32, _u32, ctypes.c_char_p]),
     "tpd_backward": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
     "tpd_read_splat_grads": (_i32, [_vp, _vp]),
     "tpd_get_backward_times": (_i32, [_vp, _vp]),
@@ -86,6 +89,8 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_uint64)]),
     "tpd_batch_read_image_f32": (_i32, [_vp, _u32, _vp]),
     "tpd_batch_read_image_rgba8": (_i32, [_vp, _u32, _vp]),
+    "tpd_batch_present_rgba8": (_i32, [_vp, _u32, _vp, ctypes.c_uint64, _vp]),
+    "tpd_batch_save_png": (_i32, [_vp, _u32, ctypes.c_char_p]),
     "tpd_batch_device_images": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
     "tpd_batch_read_pairs": (_i32, [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32)]),
     "tpd_batch_get_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),

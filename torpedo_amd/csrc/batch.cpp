@@ -553,6 +553,20 @@ static int add_scene(tpd_ctx* c, HostScene&& hs, uint32_t* scene_id) {
     return TPD_OK;
 }
 
+// The RGBA8 image of one view of the last batched frame, for the presentation calls: device
+// pointer (valid until the next frame) and size.  The frame has finished when this returns.
+int batch_view_rgba8(tpd_ctx* c, uint32_t view, const uint8_t** d_rgba, uint32_t* width, uint32_t* height) {
+    BatchState* b = state(c);
+    int rc = validate(c, b);
+    if (rc != TPD_OK) return rc;
+    if (view >= b->h_views.size()) return fail(TPD_ERR_INVALID, "batch presentation: no such view");
+    const BatchViewDev& v = b->h_views[view];
+    *d_rgba = b->d_rgba8 + 4 * v.pix_base;
+    *width = v.cam.width;
+    *height = v.cam.height;
+    return TPD_OK;
+}
+
 }  // namespace tpd
 
 using namespace tpd;

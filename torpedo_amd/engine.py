@@ -20,6 +20,7 @@ the stage readbacks used by the parity tests.
 from __future__ import annotations
 
 import ctypes
+import os
 import warnings
 from dataclasses import dataclass
 
@@ -72,6 +73,14 @@ def from_model(ply_file) -> np.ndarray:
     finally:
         lib().tpd_free_host(recs)
     return out.reshape(count, GAUSSIAN_FLOATS)
+
+
+def write_png(rgba, path) -> None:
+    """tpd_write_png: an (H, W, 4) uint8 RGBA image as a PNG file (8 bits per channel)."""
+    img = np.ascontiguousarray(rgba, dtype=np.uint8)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError("write_png: expected an (H, W, 4) uint8 array")
+    check(lib().tpd_write_png(ptr(img), int(img.shape[1]), int(img.shape[0]), os.fsencode(path)))
 
 
 class Entity(int):
@@ -313,6 +322,16 @@ class GaussianEngine:
     def copyImageTo(self, d_dst: int, stream=None):
         check(lib().tpd_copy_image(self._ctx, ctypes.c_void_p(d_dst), None if stream is None else ctypes.c_void_p(stream)))
 
+    def presentTo(self, d_dst: int, pitch: int = 0, stream=None):
+        """The frame's RGBA8 image into a device surface of row pitch `pitch` bytes (0: 4 * width),
+        ordered on `stream` (recordTargetCopy, GaussianEngine.cpp:865-875)."""
+        check(lib().tpd_present_rgba8(self._ctx, ctypes.c_void_p(d_dst), int(pitch),
+                                      None if stream is None else ctypes.c_void_p(stream)))
+
+    def savePNG(self, path: str):
+        """Writes the frame's RGBA8 image to a PNG file."""
+        check(lib().tpd_save_png(self._ctx, os.fsencode(path)))
+
     def backward(self, d_dL_dimage: int, d_means: int, d_log_scales: int, d_quats: int, d_opacity_logits: int,
                  d_sh: int, accumulate: bool = False, stream=None):
         """tpd_backward on device pointers (gradients of the last rendered frame)."""
@@ -548,6 +567,13 @@ class GaussianEngine:
         img = np.empty((h, w, 4), np.uint8)
         check(lib().tpd_batch_read_image_rgba8(self._ctx, int(view), ptr(img)))
         return img
+
+    def presentBatchViewTo(self, view: int, d_dst: int, pitch: int = 0, stream=None):
+        check(lib().tpd_batch_present_rgba8(self._ctx, int(view), ctypes.c_void_p(d_dst), int(pitch),
+                                            None if stream is None else ctypes.c_void_p(stream)))
+
+    def saveBatchPNG(self, view: int, path: str):
+        check(lib().tpd_batch_save_png(self._ctx, int(view), os.fsencode(path)))
 
     def batchImagesPtr(self):
         p, n = ctypes.c_void_p(), ctypes.c_uint64()
