@@ -96,6 +96,10 @@ int tpd_read_tiles(tpd_ctx* ctx, uint32_t* counts, uint32_t* offsets); /* N each
  * (requires tpd_config.keep_unsorted).  capacity is the length of keys/vals. */
 int tpd_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capacity, int unsorted);
 int tpd_read_ranges(tpd_ctx* ctx, uint32_t* ranges);       /* 2 * tiles uint32: [start, end) per tile */
+/* The blend's launch order of the tiles (no counterpart in the reference, whose blend dispatch
+ * takes tiles in index order, GaussianEngine.cpp:806-863): a permutation of the tiles, classes of
+ * pair count (bit length) heaviest first, empty tiles last.  Not available for an empty scene. */
+int tpd_read_tile_order(tpd_ctx* ctx, uint32_t* order);    /* tiles uint32 */
 /* Device pointer of the FP32 image (valid until the next render or resize). */
 int tpd_device_image(tpd_ctx* ctx, float** d_rgb);
 /* Device-to-device copy of the FP32 image into d_dst (H*W*3 floats), ordered on stream. */

@@ -183,3 +183,20 @@ def test_cluster_scene_1m_1080p(eng, orc):
     ref = compare_frame(eng, orc, g, cam, 3, "clusters-1M-1920x1080")
     k = ref["keys"]
     assert np.all(k[1:] >= k[:-1])
+
+
+@pytest.mark.parametrize("w,h", [(320, 200), (1920, 1080), (4096, 2304)])
+def test_tile_order_is_a_heaviest_first_permutation(eng, orc, w, h):
+    """The blend's tile launch order (1, 8 and 36 chunks of 1024 tiles): every tile exactly once,
+    and the bit length of the tile's pair count never increases along the order."""
+    g = small_scene(30000, seed=51, log_scale_mean=-3.5)
+    cam = c1_camera(orc, w, h)
+    say(f"[forward] tile order {w}x{h}")
+    gpu_render(eng, g, cam, 3)
+    order = eng.readTileOrder().astype(np.int64)
+    r = eng.readRanges().astype(np.int64)
+    n = r[:, 1] - r[:, 0]
+    assert np.array_equal(np.sort(order), np.arange(n.size))
+    cls = np.where(n > 0, np.floor(np.log2(np.maximum(n, 1))).astype(np.int64) + 1, 0)
+    assert np.all(np.diff(cls[order]) <= 0)
+    assert n.max() > 0 and cls[order[0]] == cls.max()

@@ -56,6 +56,7 @@ SIGNATURES = {
     "tpd_read_tiles": (_i32, [_vp, _vp, _vp]),
     "tpd_read_pairs": (_i32, [_vp, _vp, _vp, _u32, _i32]),
     "tpd_read_ranges": (_i32, [_vp, _vp]),
+    "tpd_read_tile_order": (_i32, [_vp, _vp]),
     "tpd_device_image": (_i32, [_vp, ctypes.POINTER(_vp)]),
     "tpd_copy_image": (_i32, [_vp, _vp, _vp]),
     "tpd_present_rgba8": (_i32, [_vp, _vp, ctypes.c_uint64, _vp]),

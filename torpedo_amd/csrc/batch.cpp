@@ -228,7 +228,7 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_splats, b->cap_splats, it)) != hipSuccess || (e = grow(b->d_tiles, b->cap_tiles, it)) != hipSuccess ||
         (e = grow(b->d_offsets, b->cap_offsets, it)) != hipSuccess ||
         (e = grow(b->d_ranges, b->cap_ranges, std::max<size_t>(tiles, 1))) != hipSuccess ||
-        (e = grow(b->d_tile_order, b->cap_tile_order, std::max<size_t>(tiles, 1))) != hipSuccess ||
+        (e = grow(b->d_tile_order, b->cap_tile_order, std::max<size_t>(tile_order_words(tiles), 1))) != hipSuccess ||
         (e = grow(b->d_image, b->cap_image, std::max<size_t>(3 * pix, 1))) != hipSuccess ||
         (e = grow(b->d_rgba8, b->cap_rgba8, std::max<size_t>(4 * pix, 1))) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc");

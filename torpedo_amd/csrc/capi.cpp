@@ -57,7 +57,7 @@ int ensure_image_buffers(tpd_ctx* c) {
     const size_t npix = (size_t)c->cam.width * c->cam.height;
     hipError_t e;
     if ((e = grow(c->d_ranges, c->cap_ranges, std::max<size_t>(ntiles, 1))) != hipSuccess ||
-        (e = grow(c->d_tile_order, c->cap_tile_order, std::max<size_t>(ntiles, 1))) != hipSuccess)
+        (e = grow(c->d_tile_order, c->cap_tile_order, std::max<size_t>(tile_order_words(ntiles), 1))) != hipSuccess)
         return hip_fail(e, "alloc ranges");
     if ((e = grow(c->d_image, c->cap_image, std::max<size_t>(3 * npix, 1))) != hipSuccess)
         return hip_fail(e, "alloc image");
@@ -381,6 +381,14 @@ int tpd_read_ranges(tpd_ctx* c, uint32_t* ranges) {
     if (!c->rendered) return fail(TPD_ERR_STATE, "tpd_read_ranges: nothing rendered");
     const size_t ntiles = (size_t)((c->cam.width + 15) / 16) * ((c->cam.height + 15) / 16);
     return copy_back(c, ranges, c->d_ranges, sizeof(uint2) * ntiles);
+}
+
+int tpd_read_tile_order(tpd_ctx* c, uint32_t* order) {
+    if (!check_ctx(c) || !order) return fail(TPD_ERR_INVALID, "tpd_read_tile_order: bad argument");
+    if (!c->rendered) return fail(TPD_ERR_STATE, "tpd_read_tile_order: nothing rendered");
+    if (c->n == 0) return fail(TPD_ERR_STATE, "tpd_read_tile_order: an empty scene blends no tiles");
+    const size_t ntiles = (size_t)((c->cam.width + 15) / 16) * ((c->cam.height + 15) / 16);
+    return copy_back(c, order, c->d_tile_order, sizeof(uint32_t) * ntiles);
 }
 
 int tpd_synchronize(tpd_ctx* c) {

@@ -171,11 +171,12 @@ void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, c
 
 // ------------------------------------------------------------------ tile order
 
-// One block of 1024 threads: count the tiles per class (bit length of the pair count, 0 for an
-// empty tile), give each class its start (heaviest class first), then scatter every tile to its
-// class; within a class the order is the order in which waves claim slots.  Every wave combines
-// its lanes of one class into one LDS atomic.
-constexpr uint32_t kOrderThreads = 1024;
+// Tiles are ordered by class (bit length of the pair count, 0 for an empty tile), heaviest class
+// first.  Two launches of one 1024-thread block per chunk of 1024 tiles: the first counts each
+// chunk's tiles per class into hist[class * chunks + chunk]; the second gives every chunk its
+// start in each class (the tiles of all heavier classes, then those of the class in earlier
+// chunks) and scatters its tiles.  Within a chunk and class the order is the order in which waves
+// claim slots.  Every wave combines its lanes of one class into one LDS atomic.
 
 __device__ __forceinline__ uint32_t tile_class(const uint2* __restrict__ ranges, uint32_t t) {
     const uint2 r = ranges[t];
@@ -183,54 +184,81 @@ __device__ __forceinline__ uint32_t tile_class(const uint2* __restrict__ ranges,
     return n ? 32u - (uint32_t)__builtin_clz(n) : 0u;  // 0 .. 32
 }
 
-__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint2* __restrict__ ranges, uint32_t ntiles,
-                                                              const uint32_t* counters, uint32_t* __restrict__ order) {
-    __shared__ uint32_t slot[33];
-    const uint32_t tid = threadIdx.x, lane = __lane_id();
-    if (counters && counters[kCtrOverflow]) return;
-    if (tid < 33) slot[tid] = 0;
-    __syncthreads();
+// Adds, for each lane with `valid`, one to slot[cls] and returns the lane's slot value before
+// the addition (lanes of one class in lane order).
+__device__ __forceinline__ uint32_t claim_class_slot(uint32_t* slot, uint32_t cls, bool valid) {
+    const uint32_t lane = __lane_id();
     const uint64_t lt = lanemask_lt();
-    // claims count tiles (pass 0) or slots (pass 1) for the lanes of each class, one atomic per class
-    const auto claim = [&](uint32_t cls, bool valid) -> uint32_t {
-        uint64_t todo = ballot(valid);
-        uint32_t mine = 0;
-        while (todo) {
-            const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
-            const uint32_t c = __builtin_amdgcn_readlane(cls, leader);
-            const uint64_t peers = ballot(valid && cls == c) & todo;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&slot[c], (uint32_t)__popcll(peers));
-            base = __builtin_amdgcn_readlane(base, leader);
-            if ((peers >> lane) & 1ull) mine = base + (uint32_t)__popcll(peers & lt);
-            todo &= ~peers;
+    uint64_t todo = ballot(valid);
+    uint32_t mine = 0;
+    while (todo) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t c = __builtin_amdgcn_readlane(cls, leader);
+        const uint64_t peers = ballot(valid && cls == c) & todo;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&slot[c], (uint32_t)__popcll(peers));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if ((peers >> lane) & 1ull) mine = base + (uint32_t)__popcll(peers & lt);
+        todo &= ~peers;
+    }
+    return mine;
+}
+
+__global__ __launch_bounds__(kOrderChunk) void k_tile_class_count(const uint2* __restrict__ ranges, uint32_t ntiles,
+                                                                  const uint32_t* counters,
+                                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kTileClasses];
+    const uint32_t tid = threadIdx.x, chunks = gridDim.x;
+    if (counters && counters[kCtrOverflow]) return;
+    if (tid < kTileClasses) cnt[tid] = 0;
+    __syncthreads();
+    const uint32_t t = blockIdx.x * kOrderChunk + tid;
+    (void)claim_class_slot(cnt, t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
+    __syncthreads();
+    if (tid < kTileClasses) hist[tid * chunks + blockIdx.x] = cnt[tid];
+}
+
+__global__ __launch_bounds__(kOrderChunk) void k_tile_order(const uint2* __restrict__ ranges, uint32_t ntiles,
+                                                            const uint32_t* counters, const uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ order) {
+    __shared__ uint32_t slot[kTileClasses], total[kTileClasses];
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave, chunks = gridDim.x;
+    if (counters && counters[kCtrOverflow]) return;
+    for (uint32_t c = wave; c < kTileClasses; c += kOrderChunk / kWave) {
+        uint64_t all = 0, before = 0;
+        for (uint32_t k = lane; k < chunks; k += kWave) {
+            const uint32_t v = hist[c * chunks + k];
+            all += v;
+            before += k < blockIdx.x ? v : 0u;
         }
-        return mine;
-    };
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
-        const uint32_t t = t0 + tid;
-        claim(t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
+        all = wave_sum_u64(all);
+        before = wave_sum_u64(before);
+        if (lane == 0) {
+            total[c] = (uint32_t)all;
+            slot[c] = (uint32_t)before;
+        }
     }
     __syncthreads();
     if (tid == 0) {
         uint32_t pos = 0;
-        for (int c = 32; c >= 0; --c) {
-            const uint32_t n = slot[c];
-            slot[c] = pos;
-            pos += n;
+        for (int c = (int)kTileClasses - 1; c >= 0; --c) {
+            slot[c] += pos;
+            pos += total[c];
         }
     }
     __syncthreads();
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
-        const uint32_t t = t0 + tid;
-        const uint32_t pos = claim(t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
-        if (t < ntiles) order[pos] = t;
-    }
+    const uint32_t t = blockIdx.x * kOrderChunk + tid;
+    const uint32_t pos = claim_class_slot(slot, t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
+    if (t < ntiles) order[pos] = t;
 }
 
 void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* counters, uint32_t* order, hipStream_t s) {
     if (ntiles == 0) return;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, ranges, ntiles, counters, order);
+    const uint32_t chunks = (ntiles + kOrderChunk - 1) / kOrderChunk;
+    uint32_t* hist = order + ntiles;
+    hipLaunchKernelGGL(k_tile_class_count, dim3(chunks), dim3(kOrderChunk), 0, s, ranges, ntiles, counters, hist);
+    hipLaunchKernelGGL(k_tile_order, dim3(chunks), dim3(kOrderChunk), 0, s, ranges, ntiles, counters,
+                       (const uint32_t*)hist, order);
 }
 
 // ------------------------------------------------------------------ blend

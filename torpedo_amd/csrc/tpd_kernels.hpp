@@ -93,7 +93,13 @@ void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, c
 void launch_blend(const BlendArgs& a, hipStream_t s);
 // The tiles in launch order for the blend: classes of pair count (bit length) from the heaviest
 // down, empty tiles last, so that the longest tiles start first and the short ones fill the tail.
-// counters (optional): nothing is done after a pair overflow.
+// counters (optional): nothing is done after a pair overflow.  `order` holds
+// tile_order_words(ntiles) words: the order, then the per-chunk class counts.
+constexpr uint32_t kOrderChunk = 1024;  // tiles per block
+constexpr uint32_t kTileClasses = 33;   // pair-count bit lengths 0 .. 32
+inline size_t tile_order_words(size_t ntiles) {
+    return ntiles + kTileClasses * ((ntiles + kOrderChunk - 1) / kOrderChunk);
+}
 void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* counters, uint32_t* order, hipStream_t s);
 
 // ------------------------------------------------------------------ batched rendering

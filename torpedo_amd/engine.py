@@ -412,6 +412,13 @@ class GaussianEngine:
         check(lib().tpd_read_ranges(self._ctx, ptr(r)))
         return r
 
+    def readTileOrder(self) -> np.ndarray:
+        """The blend's launch order of the tiles (heaviest class of pair count first)."""
+        gx, gy = (self._w + 15) // 16, (self._h + 15) // 16
+        o = np.empty(gx * gy, np.uint32)
+        check(lib().tpd_read_tile_order(self._ctx, ptr(o)))
+        return o
+
     def autotune(self) -> dict:
         """Times every radix-pass block shape on the last frame's pairs and keeps the fastest."""
         ms = np.zeros(64, np.float32)
