@@ -1,7 +1,8 @@
 // Host-side check of tpd::splat_sub_block_mask (torpedo_amd/csrc/tpd_device.hpp): for random
 // splats and tiles, every pixel centre that could pass the blend's alpha >= 1/255 test (power and
 // alpha evaluated as the blend does, exp in double with a 1e-6 relative slack for the float exp's
-// error) must lie in a sub-block whose bit is set.  Prints counts; exit status 1 on a violation.
+// error) must lie in a sub-block whose bit is set.  Usage: sub_block_mask [splats] [relative
+// error of the mask's sqrt / reciprocal / log].  Prints counts; exit status 1 on a violation.
 #include "tpd_device.hpp"
 
 #include <cmath>
@@ -11,6 +12,9 @@
 
 int main(int argc, char** argv) {
     const int n = argc > 1 ? atoi(argv[1]) : 200000;
+    // relative error given to the mask's square roots, reciprocals and logarithm (the device uses
+    // approximate instructions): 0, or +-2^-21, larger than theirs
+    tpd::mask_host_error = argc > 2 ? (float)atof(argv[2]) : 0.0f;
     std::mt19937_64 rng(12345);
     std::uniform_real_distribution<double> U(0.0, 1.0);
     long long violations = 0, bits = 0, needed = 0, splats_some = 0;

@@ -321,15 +321,46 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
 // v* (and -v*) into the band gives the band's extent.  A slack of 0.01 px + 2^-20 |coordinate|
 // covers the rounding of these few operations.  Opacity below 1/255 reaches nothing; a conic that
 // is not safely positive definite (b^2 >= 0.98 ac, NaN) may reach everything.
+//
+// The square roots, reciprocals and the logarithm are the hardware's approximate instructions
+// on the device (errors of a few ulp, against 16 and 12 instructions for the correctly rounded
+// square root and division).  Such errors move the band extents by a few parts in 2^23, inside the
+// slack; the host build lets tests/cpp/sub_block_mask.cpp run these operations with relative
+// errors of 2^-21 in either direction (mask_host_error) and check each variant against brute force.
+inline float mask_host_error = 0.0f;  // host variable: relative error given to the operations below
+__host__ __device__ inline float mask_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x) * (1.0f + mask_host_error);
+#endif
+}
+__host__ __device__ inline float mask_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return (1.0f / x) * (1.0f + mask_host_error);
+#endif
+}
+__host__ __device__ inline float mask_log(float x) {  // natural logarithm
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x) * 0.693147182f;
+#else
+    const float l = logf(x);
+    return l + (fabsf(l) + 0.5f) * mask_host_error;
+#endif
+}
+
 __host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, float a, float b, float c, float o,
                                                          float ox, float oy) {
     if (o < 1.0f / 255.0f) return 0u;  // alpha <= o for power <= 0
-    const float q0 = 1.02f * (logf(255.0f * o) + 0.05f) * (2.0f / 0.99f);
+    const float q0 = 1.02f * (mask_log(255.0f * o) + 0.05f) * (2.0f / 0.99f);
     if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c) && q0 >= 0.0f)) return 0xFFFFu;
     const float det = a * c - b * b;
-    const float ex = sqrtf(q0 * c / det), ey = sqrtf(q0 * a / det);
-    const float vstar = -b * ex / c;
-    const float aq0 = a * q0, inva = 1.0f / a;
+    const float inv_det = mask_rcp(det), inva = mask_rcp(a);
+    const float ex = mask_sqrt(q0 * c * inv_det), ey = mask_sqrt(q0 * a * inv_det);
+    const float vstar = -b * ex * mask_rcp(c);
+    const float aq0 = a * q0;
     const float sx = 0.01f + fabsf(gx) * (1.0f / 1048576.0f), sy = 0.01f + fabsf(gy) * (1.0f / 1048576.0f);
     uint32_t mask = 0;
 #pragma unroll
@@ -338,8 +369,8 @@ __host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, flo
         if (hi < -ey - sy || lo > ey + sy) continue;
         const float vlo = fmaxf(lo, -ey), vhi = fminf(hi, ey);
         const float vr = fminf(fmaxf(vstar, vlo), vhi), vl = fminf(fmaxf(-vstar, vlo), vhi);
-        const float xr = gx + (-b * vr + sqrtf(fmaxf(aq0 - det * vr * vr, 0.0f))) * inva + sx;
-        const float xl = gx + (-b * vl - sqrtf(fmaxf(aq0 - det * vl * vl, 0.0f))) * inva - sx;
+        const float xr = gx + (-b * vr + mask_sqrt(fmaxf(aq0 - det * vr * vr, 0.0f))) * inva + sx;
+        const float xl = gx + (-b * vl - mask_sqrt(fmaxf(aq0 - det * vl * vl, 0.0f))) * inva - sx;
         if (xr < ox || xl > ox + 15.0f) continue;
         const uint32_t c0 = (uint32_t)floorf(fminf(fmaxf((xl - ox) * 0.25f, 0.0f), 3.0f));
         const uint32_t c1 = (uint32_t)floorf(fminf(fmaxf((xr - ox) * 0.25f, 0.0f), 3.0f));
