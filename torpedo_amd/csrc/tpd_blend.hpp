@@ -134,7 +134,6 @@ __device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_
         const uint32_t vspread = (vrows & 1u) | ((vrows & 2u) << 3) | ((vrows & 4u) << 6) | ((vrows & 8u) << 9);
         mask = reach & (vcols * vspread);
     }
-    const uint64_t lt = lanemask_lt();
     uint64_t bal[kSubBlocks];
 #pragma unroll
     for (uint32_t q = 0; q < kSubBlocks; ++q) bal[q] = ballot((mask >> q) & 1u);
@@ -153,10 +152,12 @@ __device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_
             total += n;
         }
     }
+    // The lanes with bit q set are exactly bal[q]: it serves as the execution mask, and the
+    // lane's rank among them is a masked bit count.
 #pragma unroll
     for (uint32_t q = 0; q < kSubBlocks; ++q) {
-        if ((mask >> q) & 1u) {
-            const uint32_t off = (uint32_t)__builtin_amdgcn_readlane(start, q) + (uint32_t)__popcll(bal[q] & lt);
+        if (lane_on(bal[q])) {
+            const uint32_t off = (uint32_t)__builtin_amdgcn_readlane(start, q) + lanes_below(bal[q]);
             s.list[q][off] = (uint16_t)(tid * kSlotBytes);
         }
     }

@@ -390,6 +390,11 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_bal
 // (no vector instruction).
 __device__ __forceinline__ bool lane_on(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
+// Number of lanes below this one whose bit is set in the wave-uniform mask m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const uint32_t l = __lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
