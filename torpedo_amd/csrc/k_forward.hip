@@ -176,7 +176,10 @@ void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, c
 // chunk's tiles per class into hist[class * chunks + chunk]; the second gives every chunk its
 // start in each class (the tiles of all heavier classes, then those of the class in earlier
 // chunks) and scatters its tiles.  Within a chunk and class the order is the order in which waves
-// claim slots.  Every wave combines its lanes of one class into one LDS atomic.
+// claim slots.  Every wave combines its lanes of one class into one LDS atomic.  Inside each
+// class's range of launch positions the tiles are placed by xcd_unmap, so that each XCD (blocks
+// go to the 8 XCDs round-robin) takes one contiguous run of the class's tiles, in index order by
+// chunks: neighbouring tiles, which share splats, then mostly meet in one L2.
 
 __device__ __forceinline__ uint32_t tile_class(const uint2* __restrict__ ranges, uint32_t t) {
     const uint2 r = ranges[t];
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(kOrderChunk) void k_tile_class_count(const uint2* _
 __global__ __launch_bounds__(kOrderChunk) void k_tile_order(const uint2* __restrict__ ranges, uint32_t ntiles,
                                                             const uint32_t* counters, const uint32_t* __restrict__ hist,
                                                             uint32_t* __restrict__ order) {
-    __shared__ uint32_t slot[kTileClasses], total[kTileClasses];
+    __shared__ uint32_t slot[kTileClasses], total[kTileClasses], cstart[kTileClasses];
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave, chunks = gridDim.x;
     if (counters && counters[kCtrOverflow]) return;
     for (uint32_t c = wave; c < kTileClasses; c += kOrderChunk / kWave) {
@@ -242,14 +245,16 @@ __global__ __launch_bounds__(kOrderChunk) void k_tile_order(const uint2* __restr
     if (tid == 0) {
         uint32_t pos = 0;
         for (int c = (int)kTileClasses - 1; c >= 0; --c) {
+            cstart[c] = pos;
             slot[c] += pos;
             pos += total[c];
         }
     }
     __syncthreads();
     const uint32_t t = blockIdx.x * kOrderChunk + tid;
-    const uint32_t pos = claim_class_slot(slot, t < ntiles ? tile_class(ranges, t) : 0u, t < ntiles);
-    if (t < ntiles) order[pos] = t;
+    const uint32_t cls = t < ntiles ? tile_class(ranges, t) : 0u;
+    const uint32_t pos = claim_class_slot(slot, cls, t < ntiles);
+    if (t < ntiles) order[cstart[cls] + xcd_unmap(pos - cstart[cls], total[cls])] = t;
 }
 
 void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* counters, uint32_t* order, hipStream_t s) {

@@ -474,6 +474,20 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
     return xcd < r ? xcd * (q + 1u) + pos : r * (q + 1u) + (xcd - r) * q + pos;
 }
 
+// The inverse: the block position (0 .. nb - 1) that xcd_remap maps to logical index r.
+__host__ __device__ __forceinline__ uint32_t xcd_unmap(uint32_t r, uint32_t nb) {
+    const uint32_t q = nb / 8u, m = nb % 8u;
+    uint32_t xcd, pos;
+    if (r < m * (q + 1u)) {
+        xcd = r / (q + 1u);
+        pos = r % (q + 1u);
+    } else {
+        xcd = m + (r - m * (q + 1u)) / q;
+        pos = (r - m * (q + 1u)) % q;
+    }
+    return pos * 8u + xcd;
+}
+
 // ------------------------------------------------------------------ shared kernel pieces
 
 // VM = view * model, PVM = projView * model (Slang mul order: each entry summed k = 0..3 left to
