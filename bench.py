@@ -37,9 +37,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per pair per k_onesweep launch on this workload (8-bit digits, 512 x 12 shape; mean of
 # the frame's 2 64-bit, 1 narrowing and 3 32-bit-key passes), from two rocprofv3 --pmc runs:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
-ONESWEEP_TRAFFIC_B_PER_PAIR = 22.17
+ONESWEEP_TRAFFIC_B_PER_PAIR = 22.18
 ONESWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4: "
-                           "1292.0 MB per launch (mean of 6 passes) at 58,288,090 pairs "
+                           "1292.6 MB per launch (mean of 6 passes) at 58,288,090 pairs "
                            "(profiles/batch_frame-multiview32-clusters1M.md)")
 VALU_PEAK_TFLOPS = 157.3
 
