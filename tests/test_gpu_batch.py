@@ -325,3 +325,42 @@ def test_two_frames_in_flight(engines):
         engs[0].free(d_rec)
         for e in engs:
             e.close()
+
+
+def test_batch_depths_outside_the_sort_depth_window(engines, orc):
+    """A batch sorts on the depth-window key (tile << 27) | (depth bits - bits(2^-8)) unless a depth
+    of its layout lies outside [2^-8, 2^8).  Checked three ways, each view bit-exact with the plain
+    frame and the key shown by the pass count of a 9-bit radix shape (260 tiles: 4 passes for the
+    window key, 5 for the wider one): a layout inside the window; a new layout whose first frame
+    finds such depths; and a camera change under a kept graph that puts depths outside, which the
+    frame itself detects (it is marked overflowed and rendered again on the wider key)."""
+    from test_gpu_forward import depth_window_scene
+    beng, plain = engines
+    beng.clearScenes()
+    recs = [small_scene(6000, seed=61, log_scale_mean=-3.0), depth_window_scene("far"), depth_window_scene("near")]
+    sids = [beng.addScene(r) for r in recs]
+    degs = [3, 3, 3]
+
+    def view(sid, eye):
+        cam = orc.make_camera(list(eye), [0.0, 0.0, 0.0], [0.0, -1.0, 0.0], 320, 200, near=0.001, far=1000.0)
+        return (sid, cam.view, cam.proj_view, cam.focal_ndc, 320, 200)
+
+    beng.setSortShape(6)  # 9-bit digits
+    beng.setBatchGraph(True)
+    beng.setProfiling(True)
+    try:
+        cases = [("inside", [view(sids[0], (0.0, 0.0, -3.0))], 4),
+                 ("far depths, new layout", [view(sids[1], (0.0, 0.0, -3.0))], 5),
+                 ("near depths, new layout", [view(sids[2], (0.0, 0.0, -3.0))], 5),
+                 ("inside again", [view(sids[0], (0.0, 0.0, -3.0))], 4),
+                 ("far camera, same layout", [view(sids[0], (0.0, 0.0, -400.0))], 5)]
+        for name, views, want in cases:
+            beng.setViews(views)
+            beng.rasterBatch()
+            beng.syncBatch()
+            passes = beng.batchTimes()["radix_passes"]
+            say(f"[batch] depth window, {name}: {passes} radix passes")
+            check_batch(beng, plain, recs, degs, views, f"depth window, {name}")
+            assert passes == want, (name, passes)
+    finally:
+        beng.setProfiling(False)

@@ -200,3 +200,40 @@ def test_tile_order_is_a_heaviest_first_permutation(eng, orc, w, h):
     cls = np.where(n > 0, np.floor(np.log2(np.maximum(n, 1))).astype(np.int64) + 1, 0)
     assert np.all(np.diff(cls[order]) <= 0)
     assert n.max() > 0 and cls[order[0]] == cls.max()
+
+
+def depth_window_scene(case):
+    """4000 Gaussians around the origin plus 2000 at view depths outside [2^-8, 2^8) for a camera at
+    (0, 0, -3): behind the origin at depth ~400 ("far") or just in front of the eye at depth ~0.003
+    ("near"); scales grown / shrunk with the distance so that they still cover pixels."""
+    g = small_scene(6000, seed=61, log_scale_mean=-3.0)
+    out = g[4000:]
+    if case == "far":
+        out[:, 0:3] = out[:, 0:3] * 40.0 + np.array([0.0, 0.0, 400.0], np.float32)
+        out[:, 8:11] *= 130.0
+    else:
+        out[:, 0:3] = out[:, 0:3] * 0.0006 + np.array([0.0, 0.0, -3.0 + 0.003], np.float32)
+        out[:, 8:11] *= 0.001
+    return g
+
+
+@pytest.mark.parametrize("case", ["far", "near", "inside"])
+def test_depths_outside_the_sort_depth_window(eng, orc, case):
+    """The sort keys on (tile << 27) | (depth bits - bits(2^-8)) when every emitted depth lies in
+    [2^-8, 2^8), else on (tile << 31) | depth bits.  Frames with depths beyond either end must take
+    the wider key and stay bit-exact with the oracle; with a 9-bit radix shape the pass count shows
+    which key ran (320x200: 260 tiles = 9 bits; 9 + 27 bits in 4 passes, 9 + 31 in 5)."""
+    g = depth_window_scene(case) if case != "inside" else small_scene(6000, seed=61, log_scale_mean=-3.0)
+    cam = orc.make_camera([0.0, 0.0, -3.0], [0.0, 0.0, 0.0], [0.0, -1.0, 0.0], 320, 200, near=0.001, far=1000.0)
+    sp = orc.render(g, cam, 3)["splats"]
+    ref_depths, vis = sp["texel"][:, 2], sp["tiles"] > 0
+    outside = (ref_depths[vis] < 2.0 ** -8) | (ref_depths[vis] >= 2.0 ** 8)
+    assert outside.any() == (case != "inside"), (case, ref_depths[vis].min(), ref_depths[vis].max())
+    eng.setSortShape(6)  # 9-bit digits, 512 x 12
+    eng.setProfiling(True)
+    try:
+        compare_frame(eng, orc, g, cam, 3, f"depth window {case}")
+        passes = eng.stageTimes()["radix_passes"]
+    finally:
+        eng.setProfiling(False)
+    assert passes == (4 if case == "inside" else 5), (case, passes)

@@ -68,6 +68,11 @@ struct BatchState {
     const uint64_t* sorted_keys = nullptr;   // 64-bit keys, or 32-bit ones when key_drop != kKeysWide
     const uint32_t* sorted_vals = nullptr;
     uint32_t key_drop = kKeysWide;
+    // Sort key of the recorded frame: the depth window unless a depth of this layout fell outside
+    // it (then the positive-float form until the layout changes).
+    bool depth_wide = false;
+    uint32_t lo_bits() const { return depth_wide ? kDepthWideBits : kDepthWindowBits; }
+    uint32_t lo_base() const { return depth_wide ? 0u : kDepthBase; }
 
     // backward
     float* d_partials = nullptr; size_t cap_partials = 0;
@@ -258,6 +263,7 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
     if (e != hipSuccess) return hip_fail(e, "tpd_batch: upload layout");
     destroy_graphs(b);
     b->cap = 0;  // the next frame counts its pairs first
+    b->depth_wide = false;  // and checks its depths
     b->rendered = false;
     b->layout_dirty = false;
     b->tables_dirty = true;
@@ -348,7 +354,7 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
 }
 
 static uint32_t batch_key_bits(const BatchState* b) {
-    return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1) + 31;  // compacted keys: depth bit 31 is clear
+    return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1) + b->lo_bits();  // compacted keys
 }
 
 static BatchBlendArgs blend_args(BatchState* b) {
@@ -372,7 +378,7 @@ static BatchBlendArgs blend_args(BatchState* b) {
 // With profiling on, stage events ev[0..5] are recorded (eager frames only).
 static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool events) {
     if (events) hipEventRecord(b->ev[0], s);
-    launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, s);
+    launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, b->d_counters, s);
     const BatchArgs a = forward_args(b, keep_bwd);
     launch_batch_count(a, s);
     if (events) hipEventRecord(b->ev[1], s);
@@ -382,12 +388,12 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     b->passes = sort_shape_passes(b->ws.shape, key_bits);
     const bool in_alt = launch_sort_pairs(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
                                           &b->d_counters[kCtrPairs], b->cap, key_bits, s, events ? b->ev[3] : nullptr,
-                                          31, &b->key_drop);
+                                          b->lo_bits(), &b->key_drop, b->lo_base());
     b->sorted_keys = in_alt ? b->d_keys_alt : b->d_keys;
     b->sorted_vals = in_alt ? b->d_vals_alt : b->d_vals;
     if (events) hipEventRecord(b->ev[4], s);
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc, b->nblocks,
-                  &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, 31);
+                  &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, b->lo_bits());
     launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);
     launch_batch_blend(blend_args(b), s);
@@ -421,7 +427,7 @@ static void record_backward(BatchState* b, const float* dL, uint32_t accumulate,
 // (plus headroom), and put the count pass's state back to zero.
 static int size_pairs(tpd_ctx* c, BatchState* b) {
     hipStream_t s = c->stream;
-    launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, s);
+    launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, b->d_counters, s);
     BatchArgs a = forward_args(b, false);
     a.cap = 0;
     launch_batch_count(a, s);
@@ -433,6 +439,10 @@ static int size_pairs(tpd_ctx* c, BatchState* b) {
     if (e != hipSuccess) return hip_fail(e, "tpd_batch: count pairs");
     const uint32_t P = b->h_counters[kCtrPairs];
     if (P > kMaxSortKeys) return fail(TPD_ERR_CAPACITY, "tpd_batch: more than 2^30 - 1 tile/Gaussian pairs");
+    if (b->h_counters[kCtrDepthOut] && !b->depth_wide) {
+        b->depth_wide = true;
+        destroy_graphs(b);
+    }
     return ensure_pairs(b, std::max<size_t>((size_t)P + P / 8, 1u << 16));
 }
 
@@ -512,8 +522,18 @@ static int validate(tpd_ctx* c, BatchState* b) {
             return TPD_OK;
         }
         if (attempt > 2) return fail(TPD_ERR_STATE, "tpd_batch: pair buffers did not converge");
-        int rc = ensure_pairs(b, (size_t)P + P / 8);
-        if (rc == TPD_OK && b->pending_mode == 2) rc = ensure_partials(b, b->cap);
+        // An overflowed frame either had more pairs than the buffers hold or (sorted on the depth
+        // window) a depth outside the window; either way nothing after the sort ran.
+        const bool depth_out = !b->depth_wide && b->h_counters[kCtrDepthOut] != 0;
+        if (depth_out) {
+            b->depth_wide = true;
+            destroy_graphs(b);
+        }
+        int rc = TPD_OK;
+        if (!depth_out || P > b->cap) {
+            rc = ensure_pairs(b, (size_t)P + P / 8);
+            if (rc == TPD_OK && b->pending_mode == 2) rc = ensure_partials(b, b->cap);
+        }
         if (rc != TPD_OK) return rc;
         if ((rc = launch_frame(c, b, b->pending_mode, b->pending_dL, b->pending_accumulate, b->pending_stream)) != TPD_OK)
             return rc;
@@ -808,7 +828,7 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
         if (b->key_drop != kKeysWide) {
             // the sort left 32-bit keys: rebuild the 64-bit ones in the other (free) key buffer
             uint64_t* full = b->sorted_keys == b->d_keys ? b->d_keys_alt : b->d_keys;
-            launch_full_keys(reinterpret_cast<const uint32_t*>(b->sorted_keys), b->key_drop, 31, b->sorted_vals,
+            launch_full_keys(reinterpret_cast<const uint32_t*>(b->sorted_keys), b->key_drop, b->lo_bits(), b->sorted_vals,
                              b->d_splats, b->last_pairs, full, c->stream);
             e = hipStreamSynchronize(c->stream);
             k = full;

@@ -367,7 +367,7 @@ int tpd_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t capacity
     if (!unsorted && c->key_drop != kKeysWide && c->last_pairs) {
         // the sort left 32-bit keys: rebuild the 64-bit ones in the other (free) key buffer
         uint64_t* full = c->sorted_keys == c->d_keys ? c->d_keys_alt : c->d_keys;
-        launch_full_keys(reinterpret_cast<const uint32_t*>(c->sorted_keys), c->key_drop, 31, c->sorted_vals,
+        launch_full_keys(reinterpret_cast<const uint32_t*>(c->sorted_keys), c->key_drop, c->lo_bits, c->sorted_vals,
                          c->d_splats, c->last_pairs, full, c->stream);
         k = full;
     }

@@ -21,8 +21,10 @@
 
 namespace tpd {
 
+// Also clears the frame's depth-window flag, which the count pass that follows may set.
 __global__ void k_batch_mats(const BatchSceneDev* __restrict__ scenes, const BatchViewDev* __restrict__ views,
-                             EntityMats* __restrict__ mats) {
+                             EntityMats* __restrict__ mats, uint32_t* __restrict__ counters) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) counters[kCtrDepthOut] = 0u;
     const BatchViewDev& vw = views[blockIdx.y];
     const BatchSceneDev& sc = scenes[vw.scene];
     const uint32_t ne = sc.n_ent ? sc.n_ent : 1u;
@@ -32,10 +34,10 @@ __global__ void k_batch_mats(const BatchSceneDev* __restrict__ scenes, const Bat
 }
 
 void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, uint32_t nviews, uint32_t max_ent,
-                       EntityMats* mats, hipStream_t s) {
+                       EntityMats* mats, uint32_t* counters, hipStream_t s) {
     if (nviews == 0) return;
     const uint32_t ne = max_ent ? max_ent : 1u;
-    hipLaunchKernelGGL(k_batch_mats, dim3((ne + 63) / 64, nviews), dim3(64), 0, s, scenes, views, mats);
+    hipLaunchKernelGGL(k_batch_mats, dim3((ne + 63) / 64, nviews), dim3(64), 0, s, scenes, views, mats, counters);
 }
 
 __device__ __forceinline__ bool project_in_view(const float* rec, const EntityMats& m, const BatchViewDev& vw,
@@ -93,6 +95,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a, co
 
     // Per view: the block's pair count, stored in bases[] until the look-back gives positions.
     uint32_t my_total = 0;  // threads 0..63: sum of the counts they stored
+    bool depth_out = false;  // an emitted depth outside the sort's depth window
     for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
         const uint32_t kn = min(kWave, nviews - k0);
         for (uint32_t kk = 0; kk < kn; ++kk) {
@@ -102,6 +105,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a, co
                 Projected p;
                 project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
                 t = p.tiles;
+                depth_out |= t != 0 && depth_outside_window(p.vz);
             }
             const uint32_t ws = wave_incl_scan(t);
             if (lane == kWave - 1) s_w[wave][kk] = ws;
@@ -116,6 +120,8 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a, co
         }
         __syncthreads();
     }
+
+    if (ballot(depth_out) != 0 && lane == 0) atomicOr(&a.counters[kCtrDepthOut], 1u);
 
     if (wave == 0) {
         const uint64_t total = wave_sum_u64(my_total);

@@ -60,6 +60,10 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
 
     // Block-wide inclusive scan of tile counts.
     const uint32_t wave = tid / kWave, lane = __lane_id();
+    // Depths that will be emitted but lie outside the sort's depth window: the host then sorts the
+    // frame on the wider key.
+    if (ballot(p.tiles != 0 && depth_outside_window(p.vz)) != 0 && lane == 0)
+        atomicOr(&a.counters[kCtrDepthOut], 1u);
     const uint32_t incl_w = wave_incl_scan(p.tiles);
     if (lane == kWave - 1) s_wave_sum[wave] = incl_w;
     __syncthreads();
@@ -113,15 +117,21 @@ void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s) {
 
 // range.slang:16-35.  Empty tiles keep the (0, 0) written by the preprocess kernel (the
 // reference clears with fillBuffer, GaussianEngine.cpp:843-848).  The kernel also re-zeroes the
-// scan descriptors and the scan ticket for the next frame.
-__global__ void k_ranges(const void* keys, const uint32_t* __restrict__ counters, uint32_t cap,
+// scan descriptors and the scan ticket for the next frame.  A frame sorted on the depth-window key
+// although one of its depths lies outside the window (kCtrDepthOut) has some pairs out of depth
+// order inside their tiles (never in the wrong tile: the key's tile bits are kept); it is marked
+// overflowed, so that every later kernel skips it and the host renders it again on the wider key.
+__global__ void k_ranges(const void* keys, uint32_t* __restrict__ counters, uint32_t cap,
                          uint2* __restrict__ ranges, uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket,
                          uint32_t key_drop, uint32_t lo_bits) {
     const uint32_t P = min(counters[kCtrPairs], cap);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     for (uint32_t d = gid; d < n_scan_desc; d += nthreads) scan_desc[d] = 0;
-    if (gid == 0) *ticket = 0;
+    if (gid == 0) {
+        *ticket = 0;
+        if (lo_bits == kDepthWindowBits && counters[kCtrDepthOut]) counters[kCtrOverflow] = 1u;
+    }
     const bool wide = key_drop == kKeysWide;
     const uint32_t tshift = wide ? 32u : lo_bits - key_drop;
     const auto tile_of = [&](uint32_t i) -> uint32_t {
@@ -144,7 +154,7 @@ __global__ void k_ranges(const void* keys, const uint32_t* __restrict__ counters
     }
 }
 
-void launch_ranges(const void* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
+void launch_ranges(const void* keys, uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s, uint32_t cap,
                    uint32_t key_drop, uint32_t lo_bits) {
     uint32_t work = P_hint > ntiles ? P_hint : ntiles;

@@ -27,11 +27,13 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
     return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
 }
 
-// Keys are (high word << 32) | low word.  When the low word is known to use only lo_bits bits
-// (a positive float depth has bit 31 clear), the sort runs on the compacted key
-// (high << lo_bits) | low, which orders exactly as the full key and can need one pass less.
-__device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits) {
-    return ((k >> 32) << lo_bits) | (k & ((1ull << lo_bits) - 1ull));
+// Keys are (high word << 32) | low word.  When every low word is known to lie in
+// [lo_base, lo_base + 2^lo_bits) (a positive float depth has bit 31 clear; a depth in the window
+// has bits in [kDepthBase, kDepthBase + 2^27)), the sort runs on the compacted key
+// (high << lo_bits) | (low - lo_base), which orders exactly as the full key and can need fewer
+// passes.  The difference is masked to lo_bits bits, so the high word's bits survive any key.
+__device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits, uint32_t lo_base) {
+    return ((k >> 32) << lo_bits) | ((uint64_t)((uint32_t)k - lo_base) & ((1ull << lo_bits) - 1ull));
 }
 
 // ------------------------------------------------------------------ histograms
@@ -46,7 +48,8 @@ constexpr uint32_t kHistUnroll = 8;
 
 template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
-                                                   uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t tile_keys,
+                                                   uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t lo_base,
+                                                   uint32_t tile_keys,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
                                                    uint32_t* __restrict__ tickets) {
     constexpr uint32_t R = 1u << BITS;
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
             k[u] = i < n ? keys[i] : 0ull;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits);
+        for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits, lo_base);
 #pragma unroll
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
@@ -142,7 +145,8 @@ struct OnesweepArgs {
     const uint32_t* count_ptr;
     uint32_t n_imm;
     uint32_t shift;
-    uint32_t lo_bits;          // bits of the key's low word (32, or 31 for positive depths)
+    uint32_t lo_bits;          // bits of the key's low word (32; 31 for positive depths; 27 in the depth window)
+    uint32_t lo_base;          // subtracted from the low word before compaction
     uint32_t drop;             // kNarrow / kNarrowed: low bits of the compacted key left out of the 32-bit key
     const uint32_t* offsets;   // 2^BITS global exclusive offsets of this pass's digits
     uint32_t* desc;            // this pass's look-back region (parts x 2^BITS), zero on entry
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const auto load_key = [&](uint32_t idx) -> KT {
         if constexpr (MODE == kWide) return static_cast<const uint64_t*>(a.keys_in)[idx];
         if constexpr (MODE == kNarrow)
-            return (uint32_t)(compact_key(static_cast<const uint64_t*>(a.keys_in)[idx], a.lo_bits) >> a.drop);
+            return (uint32_t)(compact_key(static_cast<const uint64_t*>(a.keys_in)[idx], a.lo_bits, a.lo_base) >> a.drop);
         return static_cast<const uint32_t*>(a.keys_in)[idx];
     };
     KT k[ITEMS];
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     uint32_t rank[ITEMS], dig[ITEMS];
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
-        if constexpr (MODE == kWide) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits), a.shift);
+        if constexpr (MODE == kWide) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits, a.lo_base), a.shift);
         else dig[j] = (k[j] >> (a.shift - a.drop)) & ((1u << BITS) - 1u);
     }
 #pragma unroll
@@ -389,15 +393,15 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits) {
 
 template <uint32_t BITS>
 static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const uint64_t* keys, const uint32_t* count_ptr,
-                        uint32_t n_max, uint32_t lo_bits, uint32_t tile, SortWorkspace& ws) {
-    hipLaunchKernelGGL(k_sort_hist<BITS>, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, tile,
+                        uint32_t n_max, uint32_t lo_bits, uint32_t lo_base, uint32_t tile, SortWorkspace& ws) {
+    hipLaunchKernelGGL(k_sort_hist<BITS>, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, lo_base, tile,
                        ws.partials, ws.desc, ws.tickets);
     hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
 }
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop) {
+                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop, uint32_t lo_base) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -407,8 +411,8 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 
     uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
     hb = hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
-    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
-    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, tile, ws);
+    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
+    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
     // With narrow_drop, the first pass whose digit starts at or above key_bits - 32 writes 32-bit
     // keys (the bits from its digit up) and the later passes move 8 instead of 12 bytes per pair.
@@ -431,6 +435,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.count_ptr = count_ptr; o.n_imm = n_max;
         o.shift = p * sh.bits;
         o.lo_bits = lo_bits;
+        o.lo_base = lo_base;
         o.drop = narrow_pass < passes ? narrow_pass * sh.bits : 0u;
         const uint32_t mode = p < narrow_pass ? kWide : (p == narrow_pass ? kNarrow : kNarrowed);
         o.offsets = ws.offsets + p * R;

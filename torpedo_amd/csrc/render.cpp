@@ -93,13 +93,19 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
 
     bool in_alt = false;
     // Emitted depths are positive floats (view z > 0), so bit 31 of every key's low word is clear
-    // and the sort may run on the compacted (tile << 31) | depth key: same order, one bit less.
-    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 31;
+    // and the sort may run on the compacted (tile << 31) | depth key: same order, one bit less;
+    // when the preprocess saw no depth outside [2^-8, 2^8), on (tile << 27) | (depth - 2^-8 bits).
+    const bool depth_out = c->h_counters[kCtrDepthOut] != 0;
+    c->lo_bits = depth_out ? kDepthWideBits : kDepthWindowBits;
+    c->lo_base = depth_out ? 0u : kDepthBase;
+    if (depth_out && (e = hipMemsetAsync(&c->d_counters[kCtrDepthOut], 0, sizeof(uint32_t), s)) != hipSuccess)
+        return hip_fail(e, "clear depth flag");
+    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + c->lo_bits;
     c->last_passes = P > 0 ? sort_shape_passes(c->ws.shape, key_bits) : 0;
     if (c->profile) hipEventRecord(c->ev[2], s);
     if (P > 0)
         in_alt = launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, P, key_bits, s,
-                                   c->profile ? c->ev[3] : nullptr, 31, &c->key_drop);
+                                   c->profile ? c->ev[3] : nullptr, c->lo_bits, &c->key_drop, c->lo_base);
     else
         c->key_drop = kKeysWide;
     if (P == 0 && c->profile) hipEventRecord(c->ev[3], s);
@@ -114,7 +120,7 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
             return hip_fail(e, "clear evaluation counter");
     }
     launch_ranges(c->sorted_keys, c->d_counters, P, ntiles, c->d_ranges, c->d_scan_desc, nd,
-                  &c->d_counters[kCtrScanTicket], s, 0xFFFFFFFFu, c->key_drop, 31);
+                  &c->d_counters[kCtrScanTicket], s, 0xFFFFFFFFu, c->key_drop, c->lo_bits);
     launch_tile_order(c->d_ranges, ntiles, nullptr, c->d_tile_order, s);
     if (c->profile) hipEventRecord(c->ev[5], s);
     BlendArgs b{};

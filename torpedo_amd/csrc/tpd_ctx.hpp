@@ -51,6 +51,9 @@ struct tpd_ctx {
     tpd::SortWorkspace ws;
     const uint64_t* sorted_keys = nullptr;   // 64-bit keys, or 32-bit ones when key_drop != kKeysWide
     uint32_t key_drop = tpd::kKeysWide;
+    // The last frame's sort key: (tile << lo_bits) | (depth bits - lo_base), the depth window
+    // (27, kDepthBase) or the positive-float form (31, 0).
+    uint32_t lo_bits = tpd::kDepthWideBits, lo_base = 0;
     const uint32_t* sorted_vals = nullptr;
 
     // image

@@ -10,8 +10,21 @@ enum : uint32_t {
     kCtrScanTicket = 0,   // preprocess partition ticket
     kCtrPairs = 1,        // pair count P written by the last preprocess partition
     kCtrOverflow = 2,     // 1 when P exceeded the key/value capacity
+    kCtrDepthOut = 3,     // non-zero when an emitted depth lies outside the sort's depth window
     kCtrCount = 16,
 };
+
+// The sort's depth window.  Culling keeps view depths inside [near, far], and depths in
+// [2^-8, 2^8) have float bit patterns in [kDepthBase, kDepthBase + 2^27): for such frames the sort
+// may key on (tile << 27) | (depth bits - kDepthBase), in the same order as (tile << 32) | depth
+// bits and with 4 bits less than the positive-float form (tile << 31) | depth bits.  A frame with
+// any depth outside the window (flagged by its preprocess in kCtrDepthOut) sorts that 31-bit form.
+constexpr uint32_t kDepthBase = 0x3B800000u;  // floatbits(2^-8)
+constexpr uint32_t kDepthWindowBits = 27;
+constexpr uint32_t kDepthWideBits = 31;
+__device__ __forceinline__ bool depth_outside_window(float vz) {
+    return __float_as_uint(vz) - kDepthBase >= (1u << kDepthWindowBits);
+}
 
 struct ForwardArgs {
     const float* gauss;          // N x 60 floats (GaussianPoint records)
@@ -79,13 +92,15 @@ void launch_preprocess_bwd(const PreBackwardArgs& a, hipStream_t s);
 void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams& cam, EntityMats* out, hipStream_t s);
 void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
 // Sorted keys are 64-bit (tile << 32) | depth, or, after a sort that narrowed them, 32-bit
-// (compacted key (tile << lo_bits) | depth) >> drop; key_drop = kKeysWide marks the first form.
+// (compacted key (tile << lo_bits) | (depth - lo_base)) >> drop; key_drop = kKeysWide marks the
+// first form.
 constexpr uint32_t kKeysWide = 0xFFFFFFFFu;
 // Tile ranges of the sorted keys; P = min(counters[kCtrPairs], cap).  Also re-zeroes the scan
-// descriptors and the scan ticket for the next frame.
-void launch_ranges(const void* keys, const uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
+// descriptors and the scan ticket for the next frame, and sets kCtrOverflow when a frame sorted
+// on the depth-window key (lo_bits = kDepthWindowBits) has kCtrDepthOut set.
+void launch_ranges(const void* keys, uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s,
-                   uint32_t cap = 0xFFFFFFFFu, uint32_t key_drop = kKeysWide, uint32_t lo_bits = 31);
+                   uint32_t cap = 0xFFFFFFFFu, uint32_t key_drop = kKeysWide, uint32_t lo_bits = kDepthWideBits);
 // The 64-bit keys (tile << 32) | floatbits(view depth) of n sorted 32-bit keys, the tile from the
 // key and the depth from the splat of each value (for readers of the sorted pairs).
 void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, const uint32_t* vals,
@@ -195,7 +210,7 @@ struct BatchPreBwdArgs {
 };
 
 void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, uint32_t nviews, uint32_t max_ent,
-                       EntityMats* mats, hipStream_t s);
+                       EntityMats* mats, uint32_t* counters, hipStream_t s);
 void launch_batch_count(const BatchArgs& a, hipStream_t s);
 void launch_batch_preprocess(const BatchArgs& a, hipStream_t s);
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
@@ -232,13 +247,16 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits);  // passes for ke
 // from n_max.
 // Returns true when the sorted data ended in the alternate buffers.
 // after_hist (optional) is recorded between the histogram kernels and the first onesweep pass.
-// lo_bits < 32 sorts the compacted key (high word << lo_bits) | low word, valid when every key's
-// low word fits in lo_bits bits; key_bits then counts the compacted key's bits.
+// lo_bits < 32 sorts the compacted key (high word << lo_bits) | (low word - lo_base), valid when
+// every key's low word lies in [lo_base, lo_base + 2^lo_bits); key_bits then counts the compacted
+// key's bits.  A low word outside that range cannot change the high word's bits in the compacted
+// key (the difference is masked to lo_bits bits), only the order among keys of equal high word.
 // narrow_drop (optional) lets the later passes move 32-bit keys: *narrow_drop receives the number
 // of low compacted-key bits the sorted 32-bit keys leave out, or kKeysWide if the keys stayed
 // 64-bit (see launch_ranges).
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr);
+                       hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
+                       uint32_t lo_base = 0);
 
 }  // namespace tpd

@@ -34,6 +34,7 @@ static int rerun_duplicate(tpd_ctx* c, hipStream_t s) {
     const size_t nd = (c->n + kPreBlock - 1) / kPreBlock;
     hipError_t e = hipMemsetAsync(c->d_scan_desc, 0, nd * sizeof(uint64_t), s);
     if (e == hipSuccess) e = hipMemsetAsync(&c->d_counters[kCtrScanTicket], 0, sizeof(uint32_t), s);
+    if (e == hipSuccess) e = hipMemsetAsync(&c->d_counters[kCtrDepthOut], 0, sizeof(uint32_t), s);
     return e == hipSuccess ? TPD_OK : hip_fail(e, "autotune");
 }
 
@@ -61,7 +62,7 @@ int tpd_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shap
     hipStream_t s = c->stream;
     const uint32_t W = c->cam.width, H = c->cam.height;
     const uint32_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
-    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + 31;  // compacted keys, as render_forward
+    const uint32_t key_bits = bit_length(ntiles > 1 ? ntiles - 1 : 1) + c->lo_bits;  // the last frame's key
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(TPD_ERR_HIP, "tpd_autotune: hipEventCreate");
@@ -76,7 +77,7 @@ int tpd_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shap
             hipEventRecord(e0, s);
             uint32_t drop = kKeysWide;  // with the narrowed passes, as render_forward sorts
             launch_sort_pairs(c->ws, c->d_keys, c->d_vals, c->d_keys_alt, c->d_vals_alt, nullptr, c->last_pairs,
-                              key_bits, s, nullptr, 31, &drop);
+                              key_bits, s, nullptr, c->lo_bits, &drop, c->lo_base);
             hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) {
                 rc = fail(TPD_ERR_HIP, "tpd_autotune: sort failed");
