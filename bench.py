@@ -34,13 +34,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "views/s batched vs plain; fwd+bwd ms @3M Gaussians 1080p; sort % HBM peak"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per pair per k_onesweep launch on this workload (8-bit digits, 512 x 12 shape; mean of
-# the frame's 2 64-bit, 1 narrowing and 3 32-bit-key passes), from two rocprofv3 --pmc runs:
+# HBM bytes per pair per k_onesweep launch on this workload, per radix shape (mean over the frame's
+# passes: 64-bit, narrowing and 32-bit-key ones), from two rocprofv3 --pmc runs each:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
-ONESWEEP_TRAFFIC_B_PER_PAIR = 22.18
-ONESWEEP_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4: "
-                           "1292.6 MB per launch (mean of 6 passes) at 58,288,090 pairs "
-                           "(profiles/batch_frame-multiview32-clusters1M.md)")
+ONESWEEP_TRAFFIC_B_PER_PAIR = {4: 22.18, 6: 25.28}
+ONESWEEP_TRAFFIC_SOURCE = {
+    4: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4 (48-bit keys): "
+        "1292.6 MB per launch (mean of 6 passes) at 58,288,090 pairs"),
+    6: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 6: "
+        "1473.4 MB per launch (mean of 5 passes) at 58,288,090 pairs "
+        "(profiles/batch_frame-multiview32-clusters1M.md)"),
+}
 VALU_PEAK_TFLOPS = 157.3
 
 
@@ -427,10 +431,15 @@ def main():
     # per pass (key + value read and written); average launch = radix-pass time / passes.
     es = mv["eager_stage_ms"]
     ach, t_launch = onesweep_roofline(mv["pairs"], es["sort_passes"], es["radix_passes"])
+    shape_b = (mv.get("autotune") or {}).get("best_shape", args.sort_shape)
+    traffic_b = ONESWEEP_TRAFFIC_B_PER_PAIR.get(shape_b)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": round(ONESWEEP_TRAFFIC_B_PER_PAIR * mv["pairs"]),
-                "traffic_source": ONESWEEP_TRAFFIC_SOURCE, "kernel": "k_onesweep",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic_b * mv["pairs"]) if traffic_b else None,
+                "traffic_source": ONESWEEP_TRAFFIC_SOURCE.get(shape_b), "kernel": "k_onesweep",
+                "sort_shape": shape_desc(shape_b),
                 "pairs": mv["pairs"], "passes": es["radix_passes"], "avg_launch_us": round(t_launch * 1e6, 2),
+                "sort_ms": round(es["sort_histogram"] + es["sort_passes"], 4),
                 "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
     ach2, t2 = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
     ach1, _ = onesweep_roofline(P, st["sort_passes"], st["radix_passes"])
