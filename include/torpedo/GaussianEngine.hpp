@@ -103,6 +103,9 @@ public:
 
     void waitIdle() { check(tpd_synchronize(_ctx), "GaussianEngine::waitIdle"); }
 
+    // The C-ABI context, for the tpd_* calls this class does not wrap (device memory, timing).
+    [[nodiscard]] tpd_ctx* native() const noexcept { return _ctx; }
+
     // ---- readbacks (the reference cannot read its frame back)
     [[nodiscard]] uint32_t pairCount() const {
         uint32_t p = 0;

@@ -1,11 +1,14 @@
 // C++ caller of libtpd_hip.so through include/torpedo/*.hpp, written the way a caller of the
 // reference's engine is (demo/HelloGaussian/main.cpp): build a Scene of GaussianPoint groups,
-// compile it, rasterFrame a PerspectiveCamera, transform an entity, render a batch.  Writes what
-// it rendered into the directory argv[1]; tests/test_gpu_cpp_api.py checks it against the oracle.
+// compile it, rasterFrame a PerspectiveCamera, draw it into a device surface and a PNG file,
+// transform an entity, render a batch.  Writes what it rendered into the directory argv[1];
+// tests/test_gpu_cpp_api.py checks it against the oracle.
 #include <torpedo/GaussianEngine.hpp>
 
+#include <cstdint>
 #include <cstdio>
 #include <fstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -53,6 +56,22 @@ int main(int argc, char** argv) {
         dump(out + "/proj.f32", cam.getProjectionData(), 16);
         dump(out + "/image.f32", engine.readImage());
         dump(out + "/image.rgba8", engine.readRGBA8());
+        // draw(): the RGBA8 target into a device surface with a padded row pitch, as into a swap
+        // image; the padding must keep the fill byte.  savePNG(): the same image as a file.
+        {
+            const uint64_t pitch = 4 * 320 + 64, bytes = pitch * 200;
+            void* surface = nullptr;
+            if (tpd_alloc(engine.native(), bytes, &surface) != TPD_OK) throw std::runtime_error(tpd_last_error());
+            std::vector<uint8_t> fill(bytes, 0xA5);
+            if (tpd_memcpy(engine.native(), surface, fill.data(), bytes, 0) != TPD_OK)
+                throw std::runtime_error(tpd_last_error());
+            engine.draw(surface, pitch);
+            if (tpd_memcpy(engine.native(), fill.data(), surface, bytes, 1) != TPD_OK)
+                throw std::runtime_error(tpd_last_error());
+            (void)tpd_free(engine.native(), surface);
+            dump(out + "/surface.rgba8", fill);
+            engine.savePNG(out + "/frame.png");
+        }
         std::vector<uint64_t> keys;
         std::vector<uint32_t> vals;
         engine.readPairs(keys, vals);

@@ -1,6 +1,7 @@
 """The C++ API (include/torpedo/*.hpp) end to end on the GPU: tests/cpp/engine_api, built by
 build(), renders a Scene of two GaussianPoint groups and a single point through
-GaussianEngine::compile / rasterFrame, transforms one entity, renders a two-view batch and
+GaussianEngine::compile / rasterFrame, presents the frame (draw into a pitched device surface,
+savePNG), transforms one entity, renders a two-view batch and
 writes what it produced; every output is checked here against the CPU oracle (keys and values
 bit-exact, images within 1e-5) or against the plain frame it must equal."""
 import os
@@ -46,6 +47,12 @@ def test_cpp_api_matches_oracle(orc, tmp_path):
     img = load(d, "image.f32", np.float32, (200, 320, 3))
     assert np.abs(img - ref["image"]).max() <= 1e-5
     assert np.array_equal(load(d, "image.rgba8", np.uint8, (200, 320, 4)), ref["rgba8"])
+    # GaussianEngine::draw into a surface of row pitch 4 * 320 + 64, and savePNG
+    surf = load(d, "surface.rgba8", np.uint8, (200, 4 * 320 + 64))
+    assert np.array_equal(surf[:, :4 * 320].reshape(200, 320, 4), ref["rgba8"])
+    assert np.all(surf[:, 4 * 320:] == 0xA5)
+    from test_present import read_png
+    assert np.array_equal(read_png(os.path.join(d, "frame.png")), ref["rgba8"])
 
     models_t = models.copy()
     models_t[1] = load(d, "model.f32", np.float32)
