@@ -78,6 +78,8 @@ extern "C" int tpd_backward(tpd_ctx* c, const float* d_dL_dimage, float* d_grad_
         a.dL_dimg = d_dL_dimage;
         a.offsets = c->d_offsets;
         a.partials = b->partials;
+        a.final_T = c->d_final_T;
+        a.final_last = c->d_final_last;
         launch_blend_bwd(a, s);
     }
     if (c->profile) hipEventRecord(b->ev[1], s);

@@ -56,6 +56,9 @@ struct BatchState {
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
     uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
     float* d_image = nullptr; size_t cap_image = 0;
+    // per pixel final transmittance and last blended splat, written by frames kept for a backward
+    float* d_final_T = nullptr; size_t cap_final_T = 0;
+    int32_t* d_final_last = nullptr; size_t cap_final_last = 0;
     uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
 
     // pairs
@@ -122,6 +125,7 @@ void batch_release(tpd_ctx* c) {
     if (b->h_counters) (void)hipHostFree(b->h_counters);
     release(b->d_splats); release(b->d_tiles); release(b->d_offsets);
     release(b->d_ranges); release(b->d_tile_order); release(b->d_image); release(b->d_rgba8);
+    release(b->d_final_T); release(b->d_final_last);
     release(b->d_keys); release(b->d_vals); release(b->d_keys_alt); release(b->d_vals_alt);
     release(b->ws.desc); release(b->ws.partials); release(b->ws.offsets); release(b->ws.tickets);
     release(b->d_partials); release(b->d_grads);
@@ -235,6 +239,8 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_ranges, b->cap_ranges, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_tile_order, b->cap_tile_order, std::max<size_t>(tile_order_words(tiles), 1))) != hipSuccess ||
         (e = grow(b->d_image, b->cap_image, std::max<size_t>(3 * pix, 1))) != hipSuccess ||
+        (e = grow(b->d_final_T, b->cap_final_T, std::max<size_t>(pix, 1))) != hipSuccess ||
+        (e = grow(b->d_final_last, b->cap_final_last, std::max<size_t>(pix, 1))) != hipSuccess ||
         (e = grow(b->d_rgba8, b->cap_rgba8, std::max<size_t>(4 * pix, 1))) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc");
     if (!b->d_scan_desc || b->cap_scan_desc < std::max<size_t>(blocks, 1)) {
@@ -371,6 +377,8 @@ static BatchBlendArgs blend_args(BatchState* b) {
     a.rgba8 = b->d_rgba8;
     a.offsets = b->d_offsets;
     a.partials = b->d_partials;
+    a.final_T = b->d_final_T;
+    a.final_last = b->d_final_last;
     return a;
 }
 
@@ -396,7 +404,12 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
                   &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, b->lo_bits());
     launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);
-    launch_batch_blend(blend_args(b), s);
+    BatchBlendArgs ba = blend_args(b);
+    if (!keep_bwd) {  // a forward-only frame stores no per-pixel state for the backward
+        ba.final_T = nullptr;
+        ba.final_last = nullptr;
+    }
+    launch_batch_blend(ba, s);
     if (events) hipEventRecord(b->ev[6], s);
     (void)hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost, s);
 }

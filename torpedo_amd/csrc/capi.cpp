@@ -61,6 +61,9 @@ int ensure_image_buffers(tpd_ctx* c) {
         return hip_fail(e, "alloc ranges");
     if ((e = grow(c->d_image, c->cap_image, std::max<size_t>(3 * npix, 1))) != hipSuccess)
         return hip_fail(e, "alloc image");
+    if ((e = grow(c->d_final_T, c->cap_final_T, std::max<size_t>(npix, 1))) != hipSuccess ||
+        (e = grow(c->d_final_last, c->cap_final_last, std::max<size_t>(npix, 1))) != hipSuccess)
+        return hip_fail(e, "alloc pixel state");
     if ((e = grow(c->d_rgba8, c->cap_rgba8, std::max<size_t>(4 * npix, 1))) != hipSuccess)
         return hip_fail(e, "alloc rgba8");
     return TPD_OK;
@@ -186,6 +189,8 @@ int tpd_destroy(tpd_ctx* c) {
     release(c->d_tile_order);
     release(c->d_image);
     release(c->d_rgba8);
+    release(c->d_final_T);
+    release(c->d_final_last);
     release(c->d_eval_count);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     for (auto& ev : c->ev)

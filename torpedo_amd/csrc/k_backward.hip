@@ -25,7 +25,8 @@ __global__ __launch_bounds__(kTilePixels) void k_blend_bwd(const BackwardArgs a)
     __shared__ BlendBwdLds lds;
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv{tile % a.gx, tile / a.gx, a.width, a.height, a.gx, a.gy, 0};
-    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials);
+    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials, a.final_T,
+                   a.final_last);
 }
 
 void launch_blend_bwd(const BackwardArgs& a, hipStream_t s) {

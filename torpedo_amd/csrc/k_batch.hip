@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArg
     if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
-    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, nullptr);
+    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, nullptr, a.final_T, a.final_last);
 }
 
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s) {
@@ -268,7 +268,8 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlen
     if (a.counters[kCtrOverflow]) return;
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
-    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials);
+    blend_bwd_tile(lds, tv, a.ranges[tile], a.vals, a.splats, a.dL_dimg, a.offsets, a.partials, a.final_T,
+                   a.final_last);
 }
 
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s) {

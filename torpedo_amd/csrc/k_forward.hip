@@ -285,7 +285,8 @@ __global__ __launch_bounds__(kTilePixels) void k_blend(const BlendArgs a) {
     __shared__ BlendLds lds;
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv{tile % a.gx, tile / a.gx, a.width, a.height, a.gx, 0, 0};
-    blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, a.eval_count);
+    blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, a.eval_count, a.final_T,
+                      a.final_last);
 }
 
 void launch_blend(const BlendArgs& a, hipStream_t s) {

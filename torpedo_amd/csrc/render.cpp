@@ -135,6 +135,8 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     b.image = c->d_image;
     b.rgba8 = c->d_rgba8;
     b.eval_count = c->profile ? c->d_eval_count : nullptr;
+    b.final_T = c->d_final_T;  // kept for a backward of this frame
+    b.final_last = c->d_final_last;
     launch_blend(b, s);
     if (c->profile) hipEventRecord(c->ev[6], s);
     e = hipGetLastError();

@@ -184,12 +184,15 @@ __device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, cons
 //   - a wave leaves the round as soon as all its pixels have terminated.
 // Each 16-lane row of a wave walks its own sub-block's list, so one wave step evaluates up to four
 // different splats.  COUNT adds up the pixel-splat pairs whose power is evaluated (for the FLOP
-// roofline; sentinel entries are not counted).
+// roofline; sentinel entries are not counted).  With final_T / final_last, each pixel's final
+// transmittance and the sorted index of the last splat it blended (-1: none) are stored for the
+// backward, which then need not run the composite again.
 template <bool COUNT>
 __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint2 range,
                                            const uint32_t* __restrict__ vals, const Splat* __restrict__ splats,
                                            float* __restrict__ image, uint8_t* __restrict__ rgba8,
-                                           uint64_t* eval_count) {
+                                           uint64_t* eval_count, float* __restrict__ final_T = nullptr,
+                                           int32_t* __restrict__ final_last = nullptr) {
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     uint32_t pxl, pyl;
     tile_pixel(tv, pxl, pyl);
@@ -199,6 +202,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     // The per-pixel state machine runs on wave-uniform lane masks (scalar instructions) rather
     // than per-lane booleans: live = pixels not yet terminated.
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int32_t last = -1;
     uint64_t live = ballot(inside);
     uint32_t evals = 0;
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
@@ -234,6 +238,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
                 cg = cg + (C.y * alpha) * T;
                 cb = cb + (C.z * alpha) * T;
                 T = tn;
+                last = (int32_t)(base + off / kSlotBytes);
             }
             live &= ~(ok & low);
         };
@@ -268,6 +273,10 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
             q.x = unorm8(cr); q.y = unorm8(cg); q.z = unorm8(cb); q.w = 255;
             reinterpret_cast<uchar4*>(rgba8)[o] = q;
         }
+        if (final_T) {
+            final_T[o] = T;
+            final_last[o] = last;
+        }
     }
 }
 
@@ -286,10 +295,9 @@ __device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t 
     for (uint32_t q = 0; q < kSplatGrads; ++q) dst[q] = out ? out[q] : 0.0f;
 }
 
-// Backward of one tile.  Pass 1 re-runs the forward decisions to find each pixel's final
-// transmittance and last contributor (nothing is stored by the forward); pass 2 walks the tile's
-// splats back to front recovering T_i = T_{i+1} / (1 - a_i) and the colour accumulated behind
-// each splat, producing per splat dL/d(image x, y), dL/d(conic a, b, c), dL/d(rgb), dL/d(opacity).
+// Backward of one tile.  The forward frame stored each pixel's final transmittance and last
+// contributor (blend_tile's final_T / final_last); the tile's splats are walked back to front
+// recovering T_i = T_{i+1} / (1 - a_i) and the colour accumulated behind each splat, producing per splat dL/d(image x, y), dL/d(conic a, b, c), dL/d(rgb), dL/d(opacity).
 // Wave sums are combined in LDS; each (tile, splat) pair's sums are stored once, at the pair's
 // pre-sort position offsets[g] + row-major rank of the tile in g's rectangle (keygen.slang:47-53
 // order), so no global atomics are needed and the per-Gaussian sum is deterministic.  Every pair
@@ -298,7 +306,8 @@ __device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t 
 __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& tv, uint2 range,
                                                const uint32_t* __restrict__ vals, const Splat* __restrict__ splats,
                                                const float* __restrict__ dL_dimg, const uint32_t* __restrict__ offsets,
-                                               float* __restrict__ partials) {
+                                               float* __restrict__ partials, const float* __restrict__ final_T,
+                                               const int32_t* __restrict__ final_last) {
     constexpr uint32_t kG = kSplatGrads;
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     uint32_t pxl, pyl;
@@ -308,47 +317,17 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     BlendLds& s = L.s;
     if (tid == 0) L.last = -1;
     for (uint32_t x = tid; x < kTilePixels * kG; x += kTilePixels) (&L.acc[0][0])[x] = 0.0f;
-
-    // ---- pass 1: forward re-run (same decisions as blend_tile) -> T_final, last contributor
-    float T = 1.0f;
-    uint64_t live = ballot(inside);
-    int32_t last = -1;
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
-    for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
-        if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
-        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, splats);
-        if (live == 0) continue;
-        // two list entries per step, as in blend_tile
-        const auto power_of = [&](uint32_t off) {
-            const float4 A = slot4(s.a, off);
-            const float2 B = slot2(s.b, off);
-            const float dx = A.x - fx, dy = A.y - fy;
-            const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            return make_float2(power, B.y);
-        };
-        const auto rerun_one = [&](uint32_t off, float power, uint64_t act) {
-            float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
-            alpha = alpha < 0.99f ? alpha : 0.99f;
-            const float tn = T * (1.0f - alpha);
-            const uint64_t ok = act & ballot(!(alpha < 1.0f / 255.0f));
-            const uint64_t low = ballot(tn < 0.0001f);
-            if (lane_on(ok & ~low)) {
-                T = tn;
-                last = (int32_t)(base + off / kSlotBytes);
-            }
-            live &= ~(ok & low);
-        };
-        for (uint32_t i = 0; i < walk; i += 2) {
-            const uint32_t o1 = list[i], o2 = list[i + 1];
-            const float2 q1 = power_of(o1), q2 = power_of(o2);  // power, skip threshold
-            const uint64_t act1 = ballot(q1.x <= 0.0f) & ballot(!(q1.x < q1.y)) & live;
-            const uint64_t can2 = ballot(q2.x <= 0.0f) & ballot(!(q2.x < q2.y));
-            if (act1 != 0) rerun_one(o1, q1.x, act1);
-            const uint64_t act2 = can2 & live;
-            if (act2 != 0) rerun_one(o2, q2.x, act2);
-            if (live == 0) break;
-        }
+
+    // The forward frame stored each pixel's final transmittance and last blended splat.
+    float T = 1.0f;
+    int32_t last = -1;
+    const size_t o = tv.pix_base + (size_t)pyl * tv.width + pxl;
+    if (inside) {
+        T = final_T[o];
+        last = final_last[o];
     }
+    __syncthreads();  // L.last initialised
     if (last >= 0) atomicMax(&L.last, last);
     __syncthreads();
     const int32_t block_last = L.last;
@@ -360,10 +339,9 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
         store_pair_partial(tv, vals[k], splats, offsets, partials, nullptr);
     if (block_last < 0) return;  // uniform
 
-    // ---- pass 2: back to front
+    // ---- back to front
     float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f;
     if (inside) {
-        const size_t o = tv.pix_base + (size_t)pyl * tv.width + pxl;
         g_r = dL_dimg[o * 3 + 0];
         g_g = dL_dimg[o * 3 + 1];
         g_b = dL_dimg[o * 3 + 2];

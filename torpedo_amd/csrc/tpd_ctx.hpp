@@ -61,6 +61,9 @@ struct tpd_ctx {
     uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
     float* d_image = nullptr; size_t cap_image = 0;
     uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
+    // per pixel final transmittance and last blended splat of the frame, for the backward
+    float* d_final_T = nullptr; size_t cap_final_T = 0;
+    int32_t* d_final_last = nullptr; size_t cap_final_last = 0;
 
     uint32_t last_pairs = 0;
     uint32_t last_passes = 0;

@@ -54,6 +54,8 @@ struct BlendArgs {
     float* image;            // H x W x 3 or null
     uint8_t* rgba8;          // H x W x 4 or null
     uint64_t* eval_count;    // when non-null, incremented by the number of evaluated pixel-splat pairs
+    float* final_T;          // H x W final transmittance, for the backward (or null)
+    int32_t* final_last;     // H x W sorted index of the last blended splat, -1 for none (or null)
 };
 
 struct BackwardArgs {
@@ -66,6 +68,8 @@ struct BackwardArgs {
     const uint32_t* offsets; // per-Gaussian exclusive scan of tile counts (pre-sort pair position)
     float* partials;         // P x 9 per (tile, splat) pair at its pre-sort position: dL/d(image x, y,
                              // conic a, b, c, r, g, b, opacity); every pair is written
+    const float* final_T;    // the forward frame's per-pixel final transmittance
+    const int32_t* final_last;  // and last blended splat
 };
 
 struct PreBackwardArgs {
@@ -184,6 +188,8 @@ struct BatchBlendArgs {
     const float* dL_dimg;         // backward: concatenated H x W x 3 upstream gradients
     const uint32_t* offsets;      // backward: per item pre-sort pair offsets
     float* partials;              // backward: P x 9 per pair
+    float* final_T;               // per pixel final transmittance: written by a forward kept for the
+    int32_t* final_last;          // backward, read by it (with the last blended splat); or null
 };
 
 struct BatchGradsDev {            // one scene's parameter-gradient buffers
