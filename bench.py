@@ -362,7 +362,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); views and scenes would shard "
-              "across GPUs with no exchange, which this build does not run.", file=sys.stderr)
+              "across GPUs with no exchange, which is not implemented here.", file=sys.stderr)
         sys.exit(2)
 
     from torpedo_amd import scenes
