@@ -217,18 +217,43 @@ def depth_window_scene(case):
     return g
 
 
-@pytest.mark.parametrize("case", ["far", "near", "inside"])
+def depth_edge_scene(depths):
+    """small_scene(6000) plus one Gaussian at each given view depth on the axis of a camera at
+    (0, 0, -3) (view z = world z + 3 exactly), sized to cover a few pixels."""
+    from torpedo_amd.engine import gaussian_point
+    g = small_scene(6000, seed=61, log_scale_mean=-3.0)
+    extra = [gaussian_point([0.0, 0.0, float(np.float32(d) - np.float32(3.0))], 0.8, (0, 0, 0, 1),
+                            (0.004 * d,) * 3, np.full(48, 0.3, np.float32)) for d in depths]
+    for d, r in zip(depths, extra):
+        assert np.float32(r[2]) + np.float32(3.0) == np.float32(d)  # the view depth is exactly d
+    return np.concatenate([g, np.stack(extra).astype(np.float32)])
+
+
+@pytest.mark.parametrize("case", ["far", "near", "inside", "edges-inside", "edge-256", "edge-below"])
 def test_depths_outside_the_sort_depth_window(eng, orc, case):
     """The sort keys on (tile << 27) | (depth bits - bits(2^-8)) when every emitted depth lies in
     [2^-8, 2^8), else on (tile << 31) | depth bits.  Frames with depths beyond either end must take
     the wider key and stay bit-exact with the oracle; with a 9-bit radix shape the pass count shows
     which key ran (320x200: 260 tiles = 9 bits; 9 + 27 bits in 4 passes, 9 + 31 in 5)."""
-    g = depth_window_scene(case) if case != "inside" else small_scene(6000, seed=61, log_scale_mean=-3.0)
+    edge = 2.0 ** -8
+    if case in ("far", "near"):
+        g = depth_window_scene(case)
+    elif case == "inside":
+        g = small_scene(6000, seed=61, log_scale_mean=-3.0)
+    elif case == "edges-inside":  # the window's first and last float: [2^-8, 2^8 - 2^-16]
+        g = depth_edge_scene([edge, 256.0 - 2.0 ** -16])
+    elif case == "edge-256":  # the first float past the window
+        g = depth_edge_scene([256.0])
+    else:  # just below it: 2^-8 - 2^-22, the closest depth to 2^-8 that world z = d - 3 reaches
+        g = depth_edge_scene([edge - 2.0 ** -22])
     cam = orc.make_camera([0.0, 0.0, -3.0], [0.0, 0.0, 0.0], [0.0, -1.0, 0.0], 320, 200, near=0.001, far=1000.0)
     sp = orc.render(g, cam, 3)["splats"]
     ref_depths, vis = sp["texel"][:, 2], sp["tiles"] > 0
+    inside_case = case in ("inside", "edges-inside")
     outside = (ref_depths[vis] < 2.0 ** -8) | (ref_depths[vis] >= 2.0 ** 8)
-    assert outside.any() == (case != "inside"), (case, ref_depths[vis].min(), ref_depths[vis].max())
+    assert outside.any() == (not inside_case), (case, ref_depths[vis].min(), ref_depths[vis].max())
+    if case == "edges-inside":  # both edge Gaussians are visible and keep their exact depths
+        assert np.any(ref_depths[vis] == np.float32(2.0 ** -8)) and np.any(ref_depths[vis] == np.float32(256.0 - 2.0 ** -16))
     eng.setSortShape(6)  # 9-bit digits, 512 x 12
     eng.setProfiling(True)
     try:
@@ -236,4 +261,4 @@ def test_depths_outside_the_sort_depth_window(eng, orc, case):
         passes = eng.stageTimes()["radix_passes"]
     finally:
         eng.setProfiling(False)
-    assert passes == (4 if case == "inside" else 5), (case, passes)
+    assert passes == (4 if inside_case else 5), (case, passes)
