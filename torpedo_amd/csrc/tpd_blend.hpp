@@ -34,6 +34,7 @@ struct BlendLds {
     float4 c[kTilePixels + 1];  // colour
     uint16_t list[kSubBlocks][kTilePixels];
     uint32_t cnt[kBlendWaves][kSubBlocks];  // [staging wave][sub-block]
+    alignas(16) uint32_t done[kBlendWaves];  // per wave: all of its pixels have terminated
 };
 
 struct BlendBwdLds {
@@ -211,7 +212,13 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     StagedSplat next = load_staged(splats, load_value(vals, range.x + tid, range.y), range.x + tid, range.y);
     uint32_t g_after = load_value(vals, range.x + kTilePixels + tid, range.y);
     for (uint32_t base = range.x; base < range.y; base += kTilePixels) {
-        if (__syncthreads_count(!lane_on(live)) == (int)kTilePixels) break;
+        // The block leaves once every pixel has terminated: one flag per wave and one barrier
+        // (a wave rewrites its flag only after this round's staging barriers, so no wave can
+        // overwrite a flag another is still reading).
+        if (lane == 0) s.done[wave] = live == 0 ? 1u : 0u;
+        __syncthreads();
+        const uint4 done = *reinterpret_cast<const uint4*>(s.done);
+        if ((done.x & done.y & done.z & done.w) != 0u) break;
         const uint32_t walk = blend_stage_loaded<false>(s, nullptr, tv, next);
         next = load_staged(splats, g_after, base + kTilePixels + tid, range.y);
         g_after = load_value(vals, base + 2 * kTilePixels + tid, range.y);
