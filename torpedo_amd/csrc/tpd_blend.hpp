@@ -74,7 +74,8 @@ __device__ __forceinline__ void tile_pixel(const TileView& tv, uint32_t& px, uin
 
 // Stages splats [base, min(base + 256, end)) of the tile's sorted list in LDS and builds one
 // compacted list per 4x4 sub-block (sorted order kept) of the splats that may reach
-// alpha >= 1/255 at some pixel centre of the sub-block (splat_sub_block_mask).  Returns the number
+// alpha >= 1/255 at some pixel centre of the sub-block (sub_block_mask, with the per-splat part
+// the preprocess stored in the record).  Returns the number
 // of list entries this wave walks: the longest of its four lists (the shorter ones are padded
 // with the sentinel).  Ends with a barrier.
 // This thread's splat of a staging round: whether the slot holds one, its index and its record.
@@ -125,8 +126,9 @@ __device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_
         s.b[tid] = make_float4(c.z, sp.aux.x, c.w, 0.0f);
         s.c[tid] = sp.rgb;
         if (STORE_G) s_g[tid] = g;
-        const uint32_t reach = splat_sub_block_mask(p.x, p.y, c.x, c.y, c.z, c.w, (float)(tv.tx * kTile),
-                                                    (float)(tv.ty * kTile));
+        const uint32_t reach = sub_block_mask(p.x, p.y, c.x, c.y, c.z,
+                                              SubBlockReach{sp.aux.y, sp.aux.z, sp.aux.w, sp.rgb.w},
+                                              (float)(tv.tx * kTile), (float)(tv.ty * kTile));
         // sub-blocks with at least one pixel inside the view
         const uint32_t vx = min((tv.width - tv.tx * kTile + kSub - 1) / kSub, 4u);
         const uint32_t vy = min((tv.height - tv.ty * kTile + kSub - 1) / kSub, 4u);

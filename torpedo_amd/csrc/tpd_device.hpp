@@ -30,8 +30,8 @@ struct CameraParams {
 struct alignas(64) Splat {
     float4 xyzr;   // image x, image y, view depth, radius
     float4 conop;  // conic a, b, c, opacity
-    float4 rgb;    // colour rgb, 0
-    float4 aux;    // the blend's skip threshold log(1 / (255 opacity)) - 0.01, 0, 0, 0
+    float4 rgb;    // colour rgb; SubBlockReach::inva
+    float4 aux;    // the blend's skip threshold log(1 / (255 opacity)) - 0.01; SubBlockReach ey, v*, a q0
 };
 
 // Per-entity products view*model and projView*model, 32 floats per entity.
@@ -196,20 +196,6 @@ struct Projected {
     int32_t x0, y0, x1, y1;
 };
 
-// Writes one splat record: the projected values, or zeros for a culled Gaussian.
-__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p) {
-    Splat s;
-    if (vis) {
-        s.xyzr = make_float4(p.px, p.py, p.vz, p.radius);
-        s.conop = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
-        s.rgb = make_float4(p.r, p.g, p.b, 0.0f);
-        s.aux = make_float4(__logf(1.0f / (255.0f * p.opacity)) - 0.01f, 0.0f, 0.0f, 0.0f);
-    } else {
-        s.xyzr = s.conop = s.rgb = s.aux = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    *dst = s;
-}
-
 // project.slang:33-90 for the Gaussian record g (60 floats: position, opacity, quaternion xyzw,
 // scale xyz + modifier, sh[48]).  Returns false for culled / degenerate / tile-less Gaussians,
 // in which case p.tiles = 0 and p.radius = 0.
@@ -351,21 +337,44 @@ __host__ __device__ inline float mask_log(float x) {  // natural logarithm
 #endif
 }
 
-__host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, float a, float b, float c, float o,
-                                                         float ox, float oy) {
-    if (o < 1.0f / 255.0f) return 0u;  // alpha <= o for power <= 0
+// The tile-independent part, computed once per splat by the preprocess and kept in the splat
+// record: the ellipse's half-height ey, the height v* of its rightmost point, a q0 and 1/a.
+// ey < 0: the splat reaches nothing; ey NaN: it may reach everything.
+struct SubBlockReach {
+    float ey, vstar, aq0, inva;
+};
+
+__host__ __device__ inline SubBlockReach sub_block_reach(float a, float b, float c, float o) {
+    SubBlockReach r{-1.0f, 0.0f, 0.0f, 0.0f};
+    if (o < 1.0f / 255.0f) return r;  // alpha <= o for power <= 0
     const float q0 = 1.02f * (mask_log(255.0f * o) + 0.05f) * (2.0f / 0.99f);
-    if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c) && q0 >= 0.0f)) return 0xFFFFu;
+    if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c) && q0 >= 0.0f)) {
+        r.ey = __builtin_nanf("");
+        return r;
+    }
     const float det = a * c - b * b;
-    const float inv_det = mask_rcp(det), inva = mask_rcp(a);
-    const float ex = mask_sqrt(q0 * c * inv_det), ey = mask_sqrt(q0 * a * inv_det);
-    const float vstar = -b * ex * mask_rcp(c);
-    const float aq0 = a * q0;
+    const float inv_det = mask_rcp(det);
+    const float ex = mask_sqrt(q0 * c * inv_det);
+    r.ey = mask_sqrt(q0 * a * inv_det);
+    r.vstar = -b * ex * mask_rcp(c);
+    r.aq0 = a * q0;
+    r.inva = mask_rcp(a);
+    return r;
+}
+
+// The per-tile part: the band extents of the splat at (gx, gy) with conic (a, b, c) against the
+// tile with top-left pixel (ox, oy).
+__host__ __device__ inline uint32_t sub_block_mask(float gx, float gy, float a, float b, float c,
+                                                   const SubBlockReach& r, float ox, float oy) {
+    if (r.ey < 0.0f) return 0u;
+    if (!(r.ey == r.ey)) return 0xFFFFu;
+    const float det = a * c - b * b;
+    const float ey = r.ey, vstar = r.vstar, aq0 = r.aq0, inva = r.inva;
     const float sx = 0.01f + fabsf(gx) * (1.0f / 1048576.0f), sy = 0.01f + fabsf(gy) * (1.0f / 1048576.0f);
     uint32_t mask = 0;
 #pragma unroll
-    for (uint32_t r = 0; r < 4; ++r) {
-        const float lo = (oy + 4.0f * (float)r) - gy, hi = lo + 3.0f;
+    for (uint32_t band = 0; band < 4; ++band) {
+        const float lo = (oy + 4.0f * (float)band) - gy, hi = lo + 3.0f;
         if (hi < -ey - sy || lo > ey + sy) continue;
         const float vlo = fmaxf(lo, -ey), vhi = fminf(hi, ey);
         const float vr = fminf(fmaxf(vstar, vlo), vhi), vl = fminf(fmaxf(-vstar, vlo), vhi);
@@ -374,10 +383,32 @@ __host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, flo
         if (xr < ox || xl > ox + 15.0f) continue;
         const uint32_t c0 = (uint32_t)floorf(fminf(fmaxf((xl - ox) * 0.25f, 0.0f), 3.0f));
         const uint32_t c1 = (uint32_t)floorf(fminf(fmaxf((xr - ox) * 0.25f, 0.0f), 3.0f));
-        mask |= (((2u << c1) - 1u) & ~((1u << c0) - 1u)) << (4u * r);
+        mask |= (((2u << c1) - 1u) & ~((1u << c0) - 1u)) << (4u * band);
     }
     return mask;
 }
+
+__host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, float a, float b, float c, float o,
+                                                         float ox, float oy) {
+    return sub_block_mask(gx, gy, a, b, c, sub_block_reach(a, b, c, o), ox, oy);
+}
+
+// Writes one splat record: the projected values (and the tile-independent part of the blend's
+// sub-block mask), or zeros for a culled Gaussian.
+__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p) {
+    Splat s;
+    if (vis) {
+        const SubBlockReach r = sub_block_reach(p.con_a, p.con_b, p.con_c, p.opacity);
+        s.xyzr = make_float4(p.px, p.py, p.vz, p.radius);
+        s.conop = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
+        s.rgb = make_float4(p.r, p.g, p.b, r.inva);
+        s.aux = make_float4(__logf(1.0f / (255.0f * p.opacity)) - 0.01f, r.ey, r.vstar, r.aq0);
+    } else {
+        s.xyzr = s.conop = s.rgb = s.aux = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    *dst = s;
+}
+
 
 // ------------------------------------------------------------------ wave64 helpers
 
