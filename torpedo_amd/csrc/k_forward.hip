@@ -115,6 +115,54 @@ void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s) {
 
 // ------------------------------------------------------------------ ranges
 
+// Each thread takes runs of kRangeRun consecutive sorted keys, read with 16-byte loads, and
+// compares each key's tile with the one before it (the key before the run is one more load).
+constexpr uint32_t kRangeRun = 8;
+
+template <typename KT>
+__device__ __forceinline__ uint32_t range_tile(KT k, uint32_t tshift) {
+    return (uint32_t)(k >> tshift);
+}
+
+template <typename KT>
+__device__ __forceinline__ void ranges_of_run(const KT* __restrict__ keys, uint32_t i0, uint32_t P, uint32_t tshift,
+                                              uint32_t* __restrict__ r) {
+    uint32_t t[kRangeRun];
+    const uint32_t n = min(kRangeRun, P - i0);
+    if (n == kRangeRun) {
+        const uint4* v = reinterpret_cast<const uint4*>(keys + i0);  // i0 is a multiple of 8 keys
+        if constexpr (sizeof(KT) == 4) {
+            const uint4 x = v[0], y = v[1];
+            t[0] = x.x >> tshift; t[1] = x.y >> tshift; t[2] = x.z >> tshift; t[3] = x.w >> tshift;
+            t[4] = y.x >> tshift; t[5] = y.y >> tshift; t[6] = y.z >> tshift; t[7] = y.w >> tshift;
+        } else {  // 64-bit keys: the tile is the high word
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint4 x = v[q];
+                t[2 * q] = x.y;
+                t[2 * q + 1] = x.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kRangeRun; ++j) t[j] = j < n ? range_tile(keys[i0 + j], tshift) : 0u;
+    }
+    uint32_t prev = i0 > 0 ? range_tile(keys[i0 - 1], tshift) : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < kRangeRun; ++j) {
+        if (j >= n) break;
+        const uint32_t idx = i0 + j, cur = t[j];
+        if (idx == 0) {
+            r[2 * cur] = 0;
+        } else if (cur != prev) {
+            r[2 * prev + 1] = idx;
+            r[2 * cur] = idx;
+        }
+        if (idx == P - 1) r[2 * cur + 1] = P;
+        prev = cur;
+    }
+}
+
 // range.slang:16-35.  Empty tiles keep the (0, 0) written by the preprocess kernel (the
 // reference clears with fillBuffer, GaussianEngine.cpp:843-848).  The kernel also re-zeroes the
 // scan descriptors and the scan ticket for the next frame.  A frame sorted on the depth-window key
@@ -132,32 +180,22 @@ __global__ void k_ranges(const void* keys, uint32_t* __restrict__ counters, uint
         *ticket = 0;
         if (lo_bits == kDepthWindowBits && counters[kCtrDepthOut]) counters[kCtrOverflow] = 1u;
     }
-    const bool wide = key_drop == kKeysWide;
-    const uint32_t tshift = wide ? 32u : lo_bits - key_drop;
-    const auto tile_of = [&](uint32_t i) -> uint32_t {
-        return wide ? (uint32_t)(static_cast<const uint64_t*>(keys)[i] >> 32)
-                    : static_cast<const uint32_t*>(keys)[i] >> tshift;
-    };
     uint32_t* r = reinterpret_cast<uint32_t*>(ranges);
-    for (uint32_t idx = gid; idx < P; idx += nthreads) {
-        const uint32_t cur = tile_of(idx);
-        if (idx == 0) {
-            r[2 * cur] = 0;
-        } else {
-            const uint32_t prev = tile_of(idx - 1);
-            if (cur != prev) {
-                r[2 * prev + 1] = idx;
-                r[2 * cur] = idx;
-            }
-        }
-        if (idx == P - 1) r[2 * cur + 1] = P;
+    const uint32_t runs = (P + kRangeRun - 1) / kRangeRun;
+    if (key_drop == kKeysWide) {
+        for (uint32_t run = gid; run < runs; run += nthreads)
+            ranges_of_run(static_cast<const uint64_t*>(keys), run * kRangeRun, P, 32u, r);
+    } else {
+        for (uint32_t run = gid; run < runs; run += nthreads)
+            ranges_of_run(static_cast<const uint32_t*>(keys), run * kRangeRun, P, lo_bits - key_drop, r);
     }
 }
 
 void launch_ranges(const void* keys, uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s, uint32_t cap,
                    uint32_t key_drop, uint32_t lo_bits) {
-    uint32_t work = P_hint > ntiles ? P_hint : ntiles;
+    (void)ntiles;
+    uint32_t work = (P_hint + kRangeRun - 1) / kRangeRun;
     work = work > n_scan_desc ? work : n_scan_desc;
     uint32_t nb = (work + 255) / 256;
     nb = nb < 1 ? 1 : (nb > 8192 ? 8192 : nb);
