@@ -46,6 +46,16 @@ __device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits, ui
 // (kMaxPasses x 2^BITS, zero on entry; k_sort_offsets re-zeroes them after use).
 constexpr uint32_t kHistUnroll = 8;
 
+// The key of the lane below (lane 0 keeps its own) by a wave-wide DPP shift: register moves, no
+// LDS traffic (a shuffle per digit and pass goes through the LDS crossbar).
+__device__ __forceinline__ uint64_t key_of_lane_below(uint64_t k) {
+    constexpr int kWaveShr1 = 0x138;  // DPP control wave_shr:1
+    const int lo = (int)(uint32_t)k, hi = (int)(uint32_t)(k >> 32);
+    const uint32_t plo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, kWaveShr1, 0xF, 0xF, false);
+    const uint32_t phi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, kWaveShr1, 0xF, 0xF, false);
+    return ((uint64_t)phi << 32) | plo;
+}
+
 template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
                                                    uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t lo_base,
@@ -79,11 +89,12 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
             const uint32_t i = c0 + u * 256 + tid;
             const bool valid = i < n;
             const uint64_t vmask = ballot(valid);
+            const uint64_t below = key_of_lane_below(k[u]);
 #pragma unroll
             for (uint32_t p = 0; p < kMaxPasses; ++p) {
                 if (p >= passes) break;
                 const uint32_t d = digit_of<BITS>(k[u], p * BITS);
-                const uint32_t prev = __shfl_up(d, 1, 64);
+                const uint32_t prev = digit_of<BITS>(below, p * BITS);
                 const bool head = valid && (lane == 0 || prev != d);
                 const uint64_t heads = ballot(head) | ~vmask;  // invalid lanes end runs
                 if (head) {
