@@ -165,8 +165,8 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
                                                                 const uint32_t* __restrict__ scene_views,
                                                                 const uint32_t* __restrict__ block_scene,
                                                                 const EntityMats* __restrict__ mats) {
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
-    __shared__ uint32_t s_incl[kPreBlock];
+    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
+    __shared__ uint32_t s_big_incl[kPreBlock], s_start[kPreBlock];
     __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
     __shared__ uint32_t s_depth[kPreBlock];
 
@@ -203,30 +203,39 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
             if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) store_splat(a.splats + item, true, p);
             if (a.tiles) a.tiles[item] = p.tiles;
         }
-        // block-wide inclusive scan of tile counts
-        const uint32_t incl_w = wave_incl_scan(p.tiles);
-        if (lane == kWave - 1) s_wave_sum[wave] = incl_w;
+        // block-wide inclusive scans of the tile counts and of the counts above kOwnTiles
+        const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
+        const uint32_t incl_w = wave_incl_scan(t), big_w = wave_incl_scan(big);
+        if (lane == kWave - 1) {
+            s_wave_sum[wave] = incl_w;
+            s_wave_big[wave] = big_w;
+        }
         __syncthreads();
-        uint32_t wave_off = 0, block_total = 0;
+        uint32_t wave_off = 0, big_off = 0, big_total = 0;
 #pragma unroll
         for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
-            const uint32_t v = s_wave_sum[w];
-            wave_off += (w < wave) ? v : 0u;
-            block_total += v;
+            wave_off += (w < wave) ? s_wave_sum[w] : 0u;
+            const uint32_t vb = s_wave_big[w];
+            big_off += (w < wave) ? vb : 0u;
+            big_total += vb;
         }
-        const uint32_t incl = wave_off + incl_w;
-        s_incl[tid] = incl;
-        if (p.tiles) {
+        const uint32_t excl = wave_off + incl_w - t;
+        const uint32_t base = a.bases[entry0 + k];
+        s_big_incl[tid] = big_off + big_w;
+        if (big) {
+            s_start[tid] = excl;
             s_x0[tid] = p.x0;
             s_y0[tid] = p.y0;
             s_w[tid] = p.x1 - p.x0;
             s_depth[tid] = as_uint(p.vz);
+        } else if (t) {
+            emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), (uint64_t)base + excl, vw.gx, vw.tile_base, a.cap,
+                           a.keys, a.vals, item);
         }
-        const uint32_t base = a.bases[entry0 + k];
-        if (a.offsets && valid) a.offsets[item] = base + incl - p.tiles;
+        if (a.offsets && valid) a.offsets[item] = base + excl;
         __syncthreads();
-        emit_block_pairs(s_incl, s_x0, s_y0, s_w, s_depth, block_total, base, vw.gx, vw.tile_base, a.cap, a.keys,
-                         a.vals, vw.item_base + lb * kPreBlock);
+        emit_large_pairs(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, vw.gx, vw.tile_base, a.cap,
+                         a.keys, a.vals, vw.item_base + lb * kPreBlock);
         __syncthreads();
     }
 }

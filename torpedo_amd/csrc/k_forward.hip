@@ -27,8 +27,8 @@ void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams&
 
 __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs a) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
-    __shared__ uint32_t s_incl[kPreBlock];
+    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
+    __shared__ uint32_t s_big_incl[kPreBlock], s_start[kPreBlock];
     __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
     __shared__ uint32_t s_depth[kPreBlock];
     __shared__ uint64_t s_base;
@@ -58,25 +58,32 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
         a.tiles[i] = p.tiles;
     }
 
-    // Block-wide inclusive scan of tile counts.
+    // Block-wide inclusive scans of the tile counts and of the counts above kOwnTiles.
     const uint32_t wave = tid / kWave, lane = __lane_id();
     // Depths that will be emitted but lie outside the sort's depth window: the host then sorts the
     // frame on the wider key.
     if (ballot(p.tiles != 0 && depth_outside_window(p.vz)) != 0 && lane == 0)
         atomicOr(&a.counters[kCtrDepthOut], 1u);
-    const uint32_t incl_w = wave_incl_scan(p.tiles);
-    if (lane == kWave - 1) s_wave_sum[wave] = incl_w;
+    const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
+    const uint32_t incl_w = wave_incl_scan(t), big_w = wave_incl_scan(big);
+    if (lane == kWave - 1) {
+        s_wave_sum[wave] = incl_w;
+        s_wave_big[wave] = big_w;
+    }
     __syncthreads();
-    uint32_t wave_off = 0, block_total = 0;
+    uint32_t wave_off = 0, block_total = 0, big_off = 0, big_total = 0;
 #pragma unroll
     for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
-        const uint32_t v = s_wave_sum[w];
+        const uint32_t v = s_wave_sum[w], vb = s_wave_big[w];
         wave_off += (w < wave) ? v : 0u;
         block_total += v;
+        big_off += (w < wave) ? vb : 0u;
+        big_total += vb;
     }
-    const uint32_t incl = wave_off + incl_w;
-    s_incl[tid] = incl;
-    if (p.tiles) {
+    const uint32_t excl_in_block = wave_off + incl_w - t;
+    s_big_incl[tid] = big_off + big_w;
+    if (big) {
+        s_start[tid] = excl_in_block;
         s_x0[tid] = p.x0;
         s_y0[tid] = p.y0;
         s_w[tid] = p.x1 - p.x0;
@@ -101,9 +108,11 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
     }
     __syncthreads();
     const uint64_t base = s_base;
-    if (i < a.n) a.offsets[i] = (uint32_t)(base + incl - p.tiles);
-
-    emit_block_pairs(s_incl, s_x0, s_y0, s_w, s_depth, block_total, base, a.gx, 0u, a.cap, a.keys, a.vals,
+    if (i < a.n) a.offsets[i] = (uint32_t)(base + excl_in_block);
+    if (t && !big)
+        emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), base + excl_in_block, a.gx, 0u, a.cap, a.keys, a.vals,
+                       a.val_base + i);
+    emit_large_pairs(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, a.gx, 0u, a.cap, a.keys, a.vals,
                      a.val_base + part * kPreBlock);
 }
 

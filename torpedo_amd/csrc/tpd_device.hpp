@@ -571,24 +571,46 @@ __device__ __forceinline__ uint64_t scan_lookback(uint64_t* desc, uint32_t part)
     }
 }
 
-// Duplicate (keygen.slang:21-54) done by the whole block: pair j of the block belongs to the
-// thread whose inclusive tile-count offset s_incl first exceeds j, and covers that thread's
-// rectangle row-major.  Writes key ((tile_base + tile) << 32) | depth bits and value
-// val_first + owner at position base + j when below cap.
-__device__ __forceinline__ void emit_block_pairs(const uint32_t* s_incl, const int32_t* s_x0, const int32_t* s_y0,
-                                                 const int32_t* s_w, const uint32_t* s_depth, uint32_t block_total,
-                                                 uint64_t base, uint32_t gx, uint32_t tile_base, uint64_t cap,
+// Pair emission of one block of 256 Gaussians for one view (keygen.slang:21-54): Gaussian i's
+// pairs go to positions base + excl_i + k, k row-major over its tile rectangle, with key
+// ((tile_base + tile) << 32) | depth bits and value val_first + i; positions at or past cap are
+// dropped (the frame overflowed).  A Gaussian with at most kOwnTiles tiles writes its pairs from
+// its own thread (emit_own_pairs, no search); the larger ones are spread over the block
+// (emit_large_pairs): pair j of the large ones belongs to the thread whose inclusive count of large
+// tiles s_big_incl first exceeds j, which balances footprints of thousands of tiles.
+constexpr uint32_t kOwnTiles = 4;
+
+__device__ __forceinline__ void emit_own_pairs(uint32_t tiles, int32_t x0, int32_t y0, int32_t x1, uint32_t depth,
+                                               uint64_t pos, uint32_t gx, uint32_t tile_base, uint64_t cap,
+                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                               uint32_t val) {
+    int32_t x = x0, y = y0;
+    for (uint32_t k = 0; k < tiles; ++k) {
+        if (pos + k < cap) {
+            keys[pos + k] = ((uint64_t)(tile_base + (uint32_t)y * gx + (uint32_t)x) << 32) | depth;
+            vals[pos + k] = val;
+        }
+        if (++x == x1) {
+            x = x0;
+            ++y;
+        }
+    }
+}
+
+__device__ __forceinline__ void emit_large_pairs(const uint32_t* s_big_incl, const uint32_t* s_start,
+                                                 const int32_t* s_x0, const int32_t* s_y0, const int32_t* s_w,
+                                                 const uint32_t* s_depth, uint32_t big_total, uint64_t base,
+                                                 uint32_t gx, uint32_t tile_base, uint64_t cap,
                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                  uint32_t val_first) {
-    for (uint32_t j = threadIdx.x; j < block_total; j += kPreBlock) {
+    for (uint32_t j = threadIdx.x; j < big_total; j += kPreBlock) {
         uint32_t lo = 0, hi = kPreBlock - 1;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (s_incl[mid] > j) hi = mid; else lo = mid + 1;
+            if (s_big_incl[mid] > j) hi = mid; else lo = mid + 1;
         }
         const uint32_t owner = lo;
-        const uint32_t own_tiles = s_incl[owner] - (owner ? s_incl[owner - 1] : 0u);
-        const uint32_t k = j - (s_incl[owner] - own_tiles);
+        const uint32_t k = j - (owner ? s_big_incl[owner - 1] : 0u);
         const uint32_t w = (uint32_t)s_w[owner];
         // k / w through a float reciprocal, corrected by one step either way (exact below 2^22)
         uint32_t ry, rx;
@@ -602,7 +624,7 @@ __device__ __forceinline__ void emit_block_pairs(const uint32_t* s_incl, const i
             ry = k / w; rx = k - ry * w;
         }
         const uint32_t tile = tile_base + ((uint32_t)s_y0[owner] + ry) * gx + ((uint32_t)s_x0[owner] + rx);
-        const uint64_t gpos = base + j;
+        const uint64_t gpos = base + s_start[owner] + k;
         if (gpos < cap) {
             keys[gpos] = ((uint64_t)tile << 32) | s_depth[owner];
             vals[gpos] = val_first + owner;
