@@ -432,14 +432,18 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Inclusive prefix sum across the 64 lanes of a wave.
+// DPP moves only (no LDS permutes): row_shr 1, 2, 4, 8 scan each 16-lane row (lanes the shift
+// leaves without a source read 0), row_bcast15 adds row 0's total to row 1 and row 2's to row 3,
+// row_bcast31 adds the total of rows 0-1 to rows 2-3.  Every lane of the wave must be active.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t l = __lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (l >= (uint32_t)d) v += t;
-    }
-    return v;
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return (uint32_t)x;
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
