@@ -236,7 +236,10 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
         __syncthreads();
         emit_large_pairs(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, vw.gx, vw.tile_base, a.cap,
                          a.keys, a.vals, vw.item_base + lb * kPreBlock);
-        __syncthreads();
+        // No barrier before the next view: its first LDS writes (the wave sums) come after the
+        // writer passed this view's second barrier, behind every read of them, and the emission
+        // arrays are rewritten only after the next view's first barrier, which every wave reaches
+        // after its emission here.
     }
 }
 
