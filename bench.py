@@ -84,6 +84,12 @@ def shape_desc(i):
     return {"index": i, "digit_bits": b, "threads": t, "keys_per_thread": k}
 
 
+def sort_keys_per_s(P, stage_ms):
+    """Pairs sorted per second over the whole sort: histogram pass plus every radix pass."""
+    t = (stage_ms["sort_histogram"] + stage_ms["sort_passes"]) / 1000.0
+    return P / t if t > 0 else 0.0
+
+
 def onesweep_roofline(P, sort_passes_ms, passes):
     t_launch = sort_passes_ms / 1000.0 / max(passes, 1)
     achieved = 24.0 * P / t_launch / 1e9 if t_launch > 0 else 0.0
@@ -440,6 +446,7 @@ def main():
                 "sort_shape": shape_desc(shape_b),
                 "pairs": mv["pairs"], "passes": es["radix_passes"], "avg_launch_us": round(t_launch * 1e6, 2),
                 "sort_ms": round(es["sort_histogram"] + es["sort_passes"], 4),
+                "sort_gkeys_per_s": round(sort_keys_per_s(mv["pairs"], es) / 1e9, 3),
                 "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
     ach2, t2 = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
     ach1, _ = onesweep_roofline(P, st["sort_passes"], st["radix_passes"])
@@ -488,7 +495,10 @@ def main():
                         "autotune": st.get("autotune")},
         "sort_hbm_frac": {"multiview_32": round(ach / HBM_PEAK_GBS, 4), "pairs_multiview_32": mv["pairs"],
                           "sort_load_variant": round(ach2 / HBM_PEAK_GBS, 4), "pairs_sort_load": P2,
-                          "plain_1080p": round(ach1 / HBM_PEAK_GBS, 4), "pairs_plain_1080p": P},
+                          "plain_1080p": round(ach1 / HBM_PEAK_GBS, 4), "pairs_plain_1080p": P,
+                          "gkeys_per_s": {"multiview_32": round(sort_keys_per_s(mv["pairs"], es) / 1e9, 3),
+                                          "sort_load_variant": round(sort_keys_per_s(P2, st2) / 1e9, 3),
+                                          "plain_1080p": round(sort_keys_per_s(P, st) / 1e9, 3)}},
         "roofline": roofline,
         "roofline_blend": roofline_blend,
         "fwd_bwd_3M_1080p": fb,
