@@ -52,6 +52,12 @@ int main(int argc, char** argv) {
         engine.rasterFrame(cam);
 
         dump(out + "/records.f32", scene.dataAll<tpd::GaussianPoint>());
+        dump(out + "/entity_indices.u32", scene.entityIndices<tpd::GaussianPoint>());
+        dump(out + "/group_sizes.u32", scene.groupSizes<tpd::GaussianPoint>());
+        {
+            const uint32_t eb_index = scene.buildEntityMap<tpd::GaussianPoint>().at(eb);
+            dump(out + "/eb_index.u32", &eb_index, 1);
+        }
         dump(out + "/view.f32", cam.getViewMatrixData(), 16);
         dump(out + "/proj.f32", cam.getProjectionData(), 16);
         dump(out + "/image.f32", engine.readImage());

@@ -85,6 +85,27 @@ def test_scene_orders_groups_before_singles():
     assert list(s.entityIndices()) == [0, 0, 0, 1]
 
 
+def test_scene_lists_each_kind_last_added_first():
+    """Scene.h:130-162 walk single-storage EnTT views, which visit the storage's packed array from
+    its last element to its first: within groups and within singles, the last added comes first,
+    and dataAll, groupSizes, buildEntityMap and the per-point entity indices agree on that order."""
+    from torpedo_amd.engine import Scene, gaussian_point
+    s = Scene()
+    g1 = np.zeros((2, 60), np.float32)
+    g1[:, 0] = [10, 11]
+    g2 = np.zeros((3, 60), np.float32)
+    g2[:, 0] = [20, 21, 22]
+    ea = s.add(g1)
+    es1 = s.add(gaussian_point([7, 0, 0], 1.0))
+    eb = s.add(g2)
+    es2 = s.add(gaussian_point([8, 0, 0], 1.0))
+    assert (ea, es1, eb, es2) == (0, 1, 2, 3)
+    assert list(s.dataAll()[:, 0]) == [20, 21, 22, 10, 11, 8, 7]
+    assert s.groupSizes() == [3, 2]
+    assert s.buildEntityMap() == {eb: 0, ea: 1, es2: 2, es1: 3}
+    assert list(s.entityIndices()) == [0, 0, 0, 1, 1, 2, 3]
+
+
 def test_create_without_gpu_reports_no_device():
     from torpedo_amd import _capi
     lib = _capi.lib()
@@ -99,3 +120,23 @@ def test_create_without_gpu_reports_no_device():
     cfg.device = -1
     assert lib.tpd_create(ctypes.byref(h), ctypes.byref(cfg)) in (1, 7)
     assert lib.tpd_render(None, None) == 1
+
+
+def test_cpp_scene_lists_each_kind_last_added_first(tmp_path):
+    """The same order from the C++ mirror (include/torpedo/Scene.hpp), built host-only with g++."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not found")
+    exe = str(tmp_path / "scene_order")
+    cmd = [gxx, "-std=c++20", "-O1", "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+           os.path.join(ROOT, "tests", "cpp", "scene_order.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60, check=True).stdout.split("\n")
+    # entities in creation order: group (10, 11), single 7, group (20, 21, 22), single 8
+    assert out[0].split() == ["20", "21", "22", "10", "11", "8", "7"]
+    assert out[1].split() == ["3", "2"]
+    assert out[2].split() == ["1", "3", "0", "2"]
+    assert out[3].split() == ["0", "0", "0", "1", "1", "2", "3"]

@@ -32,7 +32,13 @@ def test_cpp_api_matches_oracle(orc, tmp_path):
     d = str(tmp_path)
     rec = load(d, "records.f32", np.float32, (-1, 60))
     assert rec.shape[0] == 5001
-    idx = np.concatenate([np.zeros(3000, np.uint32), np.ones(2000, np.uint32), np.full(1, 2, np.uint32)])
+    # Groups a (3000 points) then b (2000) were added, then one single point: the scene lists the
+    # last added group first (EnTT view order), so b's points come first and b is entity index 0.
+    idx = np.concatenate([np.zeros(2000, np.uint32), np.ones(3000, np.uint32), np.full(1, 2, np.uint32)])
+    assert np.array_equal(load(d, "entity_indices.u32", np.uint32), idx)
+    assert list(load(d, "group_sizes.u32", np.uint32)) == [2000, 3000]
+    eb_index = int(load(d, "eb_index.u32", np.uint32)[0])
+    assert eb_index == 0
     models = np.tile(np.eye(4, dtype=np.float32).reshape(1, 16), (3, 1))
 
     def cam(view_file, proj_file, w, h):
@@ -55,7 +61,7 @@ def test_cpp_api_matches_oracle(orc, tmp_path):
     assert np.array_equal(read_png(os.path.join(d, "frame.png")), ref["rgba8"])
 
     models_t = models.copy()
-    models_t[1] = load(d, "model.f32", np.float32)
+    models_t[eb_index] = load(d, "model.f32", np.float32)
     ref_t = orc.render(rec, c1, 3, ent_idx=idx, models=models_t)
     img_t = load(d, "image_transformed.f32", np.float32, (200, 320, 3))
     assert np.abs(img_t - ref_t["image"]).max() <= 1e-5

@@ -168,7 +168,12 @@ def test_entities_and_transforms(eng, orc):
     eng.rasterFrame(cam)
     view, pv, fo = cam.uniforms()
     ocam = orc.Camera(view, pv, fo, 320, 180)
-    models = np.stack([np.eye(4, dtype=np.float32), T1, T2]).reshape(3, 16)
+    emap = scene.buildEntityMap()
+    assert emap == {eb: 0, ea: 1, es: 2}  # the last added group first, then the single point
+    models = np.zeros((3, 16), np.float32)
+    models[emap[ea]] = np.eye(4, dtype=np.float32).reshape(16)
+    models[emap[eb]] = T1.reshape(16)
+    models[emap[es]] = T2.reshape(16)
     ref = orc.render(scene.dataAll(), ocam, 3, ent_idx=scene.entityIndices(), models=models)
     k, v = eng.readPairs()
     assert np.array_equal(k, ref["keys"]) and np.array_equal(v, ref["vals"])

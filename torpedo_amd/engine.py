@@ -89,7 +89,13 @@ class Entity(int):
 
 class Scene:
     """Scene (Scene.h:17-54): groups of points and single points; dataAll() lists groups first, then singles
-    (Scene.h:130-146), which is also the entity index order of buildEntityMap (Scene.h:154-162)."""
+    (Scene.h:130-146), which is also the entity index order of buildEntityMap (Scene.h:154-162).
+
+    The reference walks each kind with a single-storage EnTT view (EnTT v3.15.0, a build-time
+    dependency: cmake/Dependencies.cmake:9-11).  EnTT's sparse set iterates its packed array from the
+    last element to the first, and a scene only ever appends, so within each kind the most recently
+    added entity comes first; groups() and singles() give that order.
+    """
 
     def __init__(self):
         self._next = 0
@@ -119,17 +125,25 @@ class Scene:
     def countAll(self) -> int:
         return sum(g.shape[0] for _, g in self._groups) + len(self._singles)
 
+    def groups(self) -> list:
+        """(entity, points) of every group in view order (last added first)."""
+        return self._groups[::-1]
+
+    def singles(self) -> list:
+        """(entity, point) of every single point in view order (last added first)."""
+        return self._singles[::-1]
+
     def groupSizes(self) -> list[int]:
-        return [g.shape[0] for _, g in self._groups]
+        return [g.shape[0] for _, g in self.groups()]
 
     def dataAll(self) -> np.ndarray:
-        parts = [g for _, g in self._groups] + [s[None, :] for _, s in self._singles]
+        parts = [g for _, g in self.groups()] + [s[None, :] for _, s in self.singles()]
         if not parts:
             return np.zeros((0, GAUSSIAN_FLOATS), np.float32)
         return np.ascontiguousarray(np.concatenate(parts, 0), np.float32)
 
     def buildEntityMap(self) -> dict:
-        ents = [e for e, _ in self._groups] + [e for e, _ in self._singles]
+        ents = [e for e, _ in self.groups()] + [e for e, _ in self.singles()]
         return {e: i for i, e in enumerate(ents)}
 
     def entityIndices(self) -> np.ndarray:
