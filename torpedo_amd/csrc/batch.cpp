@@ -311,8 +311,8 @@ static int ensure_pairs(BatchState* b, size_t pairs) {
     const size_t cap = std::min({b->cap_keys, b->cap_vals, b->cap_keys_alt, b->cap_vals_alt, (size_t)kMaxSortKeys});
     const uint32_t parts = sort_parts((uint32_t)cap);
     if (!b->ws.partials) {
-        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
-            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
+        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
             (e = hipMalloc(&b->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
             (e = hipMalloc(&b->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
             return hip_fail(e, "tpd_batch: alloc sort workspace");

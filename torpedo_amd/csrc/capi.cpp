@@ -92,8 +92,8 @@ int ensure_sort_workspace(tpd_ctx* c, size_t pairs) {
     const uint32_t parts = sort_parts((uint32_t)std::min<size_t>(pairs, kMaxSortKeys));
     hipError_t e;
     if (!c->ws.partials) {
-        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
-            (e = hipMemset(c->ws.partials, 0, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess)
+        if ((e = hipMalloc(&c->ws.partials, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMemset(c->ws.partials, 0, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort counts");
         if ((e = hipMalloc(&c->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess)
             return hip_fail(e, "alloc sort offsets");

@@ -42,8 +42,10 @@ __device__ __forceinline__ uint64_t compact_key(uint64_t k, uint32_t lo_bits, ui
 // (all pairs of one Gaussian share its depth bits; neighbouring tiles share the high tile bits),
 // so each wave first collapses runs of equal digits (run heads found with one shuffle and one
 // ballot) and only run heads add their run length to the block's LDS histogram.  Each thread
-// keeps kHistUnroll loads in flight; blocks add their non-zero bins to the global counts
-// (kMaxPasses x 2^BITS, zero on entry; k_sort_offsets re-zeroes them after use).
+// keeps kHistUnroll loads in flight; blocks add their non-zero bins to the global counts, block b
+// into copy b mod kCountCopies (kCountCopies x kMaxPasses x kMaxRadix, zero on entry), so that
+// each counter takes 1/kCountCopies of the blocks' atomics; k_sort_offsets sums the copies and
+// re-zeroes them.
 constexpr uint32_t kHistUnroll = 8;
 
 // The key of the lane below (lane 0 keeps its own) by a wave-wide DPP shift: register moves, no
@@ -109,12 +111,12 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     for (uint32_t p = 0; p < passes; ++p)
         for (uint32_t d = tid; d < R; d += 256) {
             const uint32_t v = h[p][d];
-            if (v) atomicAdd(&counts[p * R + d], v);
+            if (v) atomicAdd(&counts[(blockIdx.x % kCountCopies) * (kMaxPasses * kMaxRadix) + p * R + d], v);
         }
 }
 
-// One block per pass: exclusive scan of the 2^BITS digit counts (each thread takes a contiguous
-// run of digits), which are then re-zeroed.
+// One block per pass: exclusive scan of the 2^BITS digit counts summed over their kCountCopies
+// copies (each thread takes a contiguous run of digits); every copy is then re-zeroed.
 template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets) {
     constexpr uint32_t R = 1u << BITS;
@@ -124,8 +126,13 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
     uint32_t c[DPT], local = 0;
 #pragma unroll
     for (uint32_t i = 0; i < DPT; ++i) {
-        c[i] = counts[p * R + t * DPT + i];
-        counts[p * R + t * DPT + i] = 0;
+        c[i] = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kCountCopies; ++k) {
+            uint32_t* q = counts + k * (kMaxPasses * kMaxRadix) + p * R + t * DPT + i;
+            c[i] += *q;
+            *q = 0;
+        }
         local += c[i];
     }
     const uint32_t incl = wave_incl_scan(local);

@@ -229,13 +229,14 @@ constexpr uint32_t kMaxRadixBits = 9;                      // digit widths 8 and
 constexpr uint32_t kMaxRadix = 1u << kMaxRadixBits;
 constexpr uint32_t kSortMinTile = 2048;                    // smallest partition of any block shape
 constexpr uint32_t kMaxPasses = 8;
+constexpr uint32_t kCountCopies = 8;  // copies of the digit counts, block b adds into copy b mod 8
 constexpr uint32_t kHistBlocks = 1024;                      // blocks of the digit-histogram kernel
 constexpr uint32_t kMaxSortKeys = (1u << 30) - 1;          // 30-bit look-back values
 
 struct SortWorkspace {
     uint32_t* desc = nullptr;      // 2 regions x parts_cap x kMaxRadix look-back descriptors
     uint32_t parts_cap = 0;
-    uint32_t* partials = nullptr;  // kMaxPasses x kMaxRadix global digit counts (zero between sorts)
+    uint32_t* partials = nullptr;  // kCountCopies x kMaxPasses x kMaxRadix digit counts (zero between sorts)
     uint32_t* offsets = nullptr;   // kMaxPasses x kMaxRadix global exclusive digit offsets
     uint32_t* tickets = nullptr;   // kMaxPasses partition tickets
     uint32_t shape = 0;            // onesweep block shape (index into the tuner's table)
