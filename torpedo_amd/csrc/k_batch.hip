@@ -165,10 +165,15 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
                                                                 const uint32_t* __restrict__ scene_views,
                                                                 const uint32_t* __restrict__ block_scene,
                                                                 const EntityMats* __restrict__ mats) {
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
-    __shared__ uint32_t s_big_incl[kPreBlock], s_start[kPreBlock];
-    __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
-    __shared__ uint32_t s_depth[kPreBlock];
+    // Views are software-pipelined so that one barrier per view suffices: the segment between two
+    // barriers emits the large splats' pairs of view k-1, derives the pair positions of view k and
+    // writes its own pairs, and projects view k+1.  The LDS arrays are double-buffered by view
+    // parity: a segment reads view k-1's emission arrays and view k's wave sums while it writes
+    // view k's emission arrays and view k+1's wave sums.
+    __shared__ uint32_t s_wave_sum[2][kPreBlock / kWave], s_wave_big[2][kPreBlock / kWave];
+    __shared__ uint32_t s_big_incl[2][kPreBlock], s_start[2][kPreBlock];
+    __shared__ int32_t s_x0[2][kPreBlock], s_y0[2][kPreBlock], s_w[2][kPreBlock];
+    __shared__ uint32_t s_depth[2][kPreBlock];
 
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     const uint32_t b = blockIdx.x;
@@ -179,6 +184,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
     const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
     const uint32_t entry0 = sc.entry_base + lb * nviews;
     const uint32_t sh_degree = sc.sh_degree;
+    if (nviews == 0) return;
 
     // The record is read once and kept in registers for every view.
     float rec[60];
@@ -194,52 +200,75 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs 
         for (int q = 0; q < 60; ++q) rec[q] = 0.0f;
     }
 
-    for (uint32_t k = 0; k < nviews; ++k) {
+    // This thread's Gaussian in the view last projected, and its wave-inclusive counts.
+    Projected p;
+    uint32_t incl_w = 0, big_w = 0;
+    // Projects view k, writes the splat record and tile count, and publishes the wave totals of
+    // the tile counts and of the counts above kOwnTiles.
+    const auto project_view = [&](uint32_t k) {
         const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
-        Projected p;
         p.tiles = 0;
         if (valid) {
             if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) store_splat(a.splats + item, true, p);
             if (a.tiles) a.tiles[item] = p.tiles;
         }
-        // block-wide inclusive scans of the tile counts and of the counts above kOwnTiles
         const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
-        const uint32_t incl_w = wave_incl_scan(t), big_w = wave_incl_scan(big);
+        incl_w = wave_incl_scan(t);
+        big_w = wave_incl_scan(big);
         if (lane == kWave - 1) {
-            s_wave_sum[wave] = incl_w;
-            s_wave_big[wave] = big_w;
+            s_wave_sum[k & 1u][wave] = incl_w;
+            s_wave_big[k & 1u][wave] = big_w;
         }
+    };
+
+    project_view(0);
+    uint32_t big_total_prev = 0, base_prev = 0;
+    for (uint32_t k = 0; k < nviews; ++k) {
         __syncthreads();
-        uint32_t wave_off = 0, big_off = 0, big_total = 0;
+        if (k > 0) {
+            const uint32_t q = (k - 1) & 1u;
+            const BatchViewDev& pv = views[scene_views[sc.view_begin + k - 1]];
+            emit_large_pairs(s_big_incl[q], s_start[q], s_x0[q], s_y0[q], s_w[q], s_depth[q], big_total_prev,
+                             base_prev, pv.gx, pv.tile_base, a.cap, a.keys, a.vals, pv.item_base + lb * kPreBlock);
+        }
+        {
+            const uint32_t q = k & 1u;
+            const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
+            const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
+            uint32_t wave_off = 0, big_off = 0, big_total = 0;
 #pragma unroll
-        for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
-            wave_off += (w < wave) ? s_wave_sum[w] : 0u;
-            const uint32_t vb = s_wave_big[w];
-            big_off += (w < wave) ? vb : 0u;
-            big_total += vb;
+            for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+                wave_off += (w < wave) ? s_wave_sum[q][w] : 0u;
+                const uint32_t vb = s_wave_big[q][w];
+                big_off += (w < wave) ? vb : 0u;
+                big_total += vb;
+            }
+            const uint32_t excl = wave_off + incl_w - t;
+            const uint32_t base = a.bases[entry0 + k];
+            s_big_incl[q][tid] = big_off + big_w;
+            if (big) {
+                s_start[q][tid] = excl;
+                s_x0[q][tid] = p.x0;
+                s_y0[q][tid] = p.y0;
+                s_w[q][tid] = p.x1 - p.x0;
+                s_depth[q][tid] = as_uint(p.vz);
+            } else if (t) {
+                emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), (uint64_t)base + excl, vw.gx, vw.tile_base, a.cap,
+                               a.keys, a.vals, vw.item_base + g);
+            }
+            if (a.offsets && valid) a.offsets[vw.item_base + g] = base + excl;
+            big_total_prev = big_total;
+            base_prev = base;
         }
-        const uint32_t excl = wave_off + incl_w - t;
-        const uint32_t base = a.bases[entry0 + k];
-        s_big_incl[tid] = big_off + big_w;
-        if (big) {
-            s_start[tid] = excl;
-            s_x0[tid] = p.x0;
-            s_y0[tid] = p.y0;
-            s_w[tid] = p.x1 - p.x0;
-            s_depth[tid] = as_uint(p.vz);
-        } else if (t) {
-            emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), (uint64_t)base + excl, vw.gx, vw.tile_base, a.cap,
-                           a.keys, a.vals, item);
-        }
-        if (a.offsets && valid) a.offsets[item] = base + excl;
-        __syncthreads();
-        emit_large_pairs(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, vw.gx, vw.tile_base, a.cap,
-                         a.keys, a.vals, vw.item_base + lb * kPreBlock);
-        // No barrier before the next view: its first LDS writes (the wave sums) come after the
-        // writer passed this view's second barrier, behind every read of them, and the emission
-        // arrays are rewritten only after the next view's first barrier, which every wave reaches
-        // after its emission here.
+        if (k + 1 < nviews) project_view(k + 1);
+    }
+    __syncthreads();
+    {
+        const uint32_t q = (nviews - 1) & 1u;
+        const BatchViewDev& pv = views[scene_views[sc.view_begin + nviews - 1]];
+        emit_large_pairs(s_big_incl[q], s_start[q], s_x0[q], s_y0[q], s_w[q], s_depth[q], big_total_prev, base_prev,
+                         pv.gx, pv.tile_base, a.cap, a.keys, a.vals, pv.item_base + lb * kPreBlock);
     }
 }
 
