@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per pair per k_onesweep launch on this workload, per radix shape (mean over the frame's
 # passes: 64-bit, narrowing and 32-bit-key ones), from two rocprofv3 --pmc runs each:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
-ONESWEEP_TRAFFIC_B_PER_PAIR = {4: 22.18, 6: 25.25}
+ONESWEEP_TRAFFIC_B_PER_PAIR = {4: 22.18, 6: 25.30}
 ONESWEEP_TRAFFIC_SOURCE = {
     4: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4 (48-bit keys): "
         "1292.6 MB per launch (mean of 6 passes) at 58,288,090 pairs"),
     6: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 6: "
-        "1471.7 MB per launch (mean of 5 passes) at 58,288,090 pairs "
+        "1474.7 MB per launch (mean of 5 passes) at 58,288,090 pairs "
         "(profiles/batch_frame-multiview32-clusters1M.md)"),
 }
 VALU_PEAK_TFLOPS = 157.3
