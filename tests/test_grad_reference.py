@@ -26,20 +26,23 @@ def float64_default():
 GROUPS = ("means", "log_scales", "quats", "opacity_logits", "sh")
 
 
-def setup_frame(orc, n=2000, w=64, h=64, seed=3, deg=3, log_scale_mean=-3.0):
+def setup_frame(orc, n=2000, w=64, h=64, seed=3, deg=3, log_scale_mean=-3.0, eye=(0, 0, -3), up=(0, -1, 0)):
     from oracle import torch_ref
     g = small_scene(n, seed=seed, log_scale_mean=log_scale_mean)
     params = torch_ref.make_params(g)
     rec = torch_ref.records_from_params(params)
-    cam = orc.make_camera([0, 0, -3], [0, 0, 0], [0, -1, 0], w, h)
+    cam = orc.make_camera(list(eye), [0, 0, 0], list(up), w, h)
     r = orc.render(rec, cam, deg)
     counts, pos = orc.blend_decisions(r["splats"], r["vals"], r["ranges"], w, h)
     return params, rec, cam, r, counts, pos
 
 
-def test_dense_forward_matches_fp32_oracle(orc):
+@pytest.mark.parametrize("eye,up", [((0, 0, -3), (0, -1, 0)), ((2.1, -1.4, -1.6), (0.3, 0.2, 1.0))])
+def test_dense_forward_matches_fp32_oracle(orc, eye, up):
+    """Also from an oblique camera (rotated view, off-axis means), where the oracle's projection is
+    pinned independently by tests/test_oracle_geometry.py."""
     from oracle import torch_ref
-    params, rec, cam, r, counts, pos = setup_frame(orc)
+    params, rec, cam, r, counts, pos = setup_frame(orc, eye=eye, up=up, w=80, h=56)
     img = torch_ref.render(params, cam, 3, r["vals"], counts, pos).detach().numpy()
     assert counts.sum() > 5000
     err = np.abs(img - r["image"]).max()
@@ -49,7 +52,8 @@ def test_dense_forward_matches_fp32_oracle(orc):
 @pytest.mark.parametrize("deg", [0, 1, 2, 3])
 def test_autograd_matches_finite_differences(orc, deg):
     from oracle import torch_ref
-    params, rec, cam, r, counts, pos = setup_frame(orc, n=600, w=48, h=48, seed=10 + deg, deg=deg)
+    eye, up = ((0, 0, -3), (0, -1, 0)) if deg % 2 == 0 else ((1.8, 1.2, -2.0), (-0.2, 1.0, 0.4))
+    params, rec, cam, r, counts, pos = setup_frame(orc, n=600, w=48, h=48, seed=10 + deg, deg=deg, eye=eye, up=up)
     rng = np.random.default_rng(deg)
     up = rng.normal(size=(48, 48, 3))
     gr, _ = torch_ref.grads(params, cam, deg, r["vals"], counts, pos, up)

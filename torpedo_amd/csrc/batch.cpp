@@ -144,16 +144,20 @@ static BatchState* state(tpd_ctx* c) {
 }
 
 
-// Waits for a launched frame without validating it (before host-side table or buffer changes).
-static int wait_pending(BatchState* b) {
+static int validate(tpd_ctx* c, BatchState* b);
+
+// Finishes a launched frame before host-side table or buffer changes, or before another frame is
+// launched over it: waits for it and, when it overflowed its pair buffers, grows them and renders
+// it again, so that an asynchronous step's images and gradients are complete even when the caller
+// never synchronised it.
+static int finish_pending(tpd_ctx* c, BatchState* b) {
     if (!b->pending) return TPD_OK;
-    hipError_t e = hipStreamSynchronize(b->pending_stream);
-    return e == hipSuccess ? TPD_OK : hip_fail(e, "batch frame");
+    return validate(c, b);
 }
 
 void batch_set_sort_shape(tpd_ctx* c, uint32_t shape) {
     if (!c->batch) return;
-    (void)wait_pending(c->batch);
+    (void)finish_pending(c, c->batch);
     c->batch->ws.shape = shape;
     destroy_graphs(c->batch);
 }
@@ -462,7 +466,7 @@ static int size_pairs(tpd_ctx* c, BatchState* b) {
 static int prepare(tpd_ctx* c, BatchState* b) {
     int rc;
     if (b->scenes.empty() || b->views.empty()) return fail(TPD_ERR_STATE, "tpd_batch: add scenes and set views first");
-    if ((rc = wait_pending(b)) != TPD_OK) return rc;
+    if ((rc = finish_pending(c, b)) != TPD_OK) return rc;
     if (b->layout_dirty && (rc = build_layout(c, b)) != TPD_OK) return rc;
     if (b->tables_dirty && (rc = upload_tables(b)) != TPD_OK) return rc;
     if (b->cap == 0 && (rc = size_pairs(c, b)) != TPD_OK) return rc;
@@ -578,7 +582,7 @@ static int upload_grads(BatchState* b, const tpd_scene_grads* grads) {
 
 static int add_scene(tpd_ctx* c, HostScene&& hs, uint32_t* scene_id) {
     BatchState* b = state(c);
-    int rc = wait_pending(b);
+    int rc = finish_pending(c, b);
     if (rc != TPD_OK) return rc;
     b->scenes.push_back(std::move(hs));
     if (scene_id) *scene_id = (uint32_t)(b->scenes.size() - 1);
@@ -668,7 +672,7 @@ int tpd_scene_set_transform(tpd_ctx* c, uint32_t scene, uint32_t entity, const f
     if (!hs.d_models || entity >= hs.n_ent)
         return fail(TPD_ERR_INVALID, "tpd_scene_set_transform: no such entity (scene added without models)");
     hipSetDevice(c->device);
-    int rc = wait_pending(b);
+    int rc = finish_pending(c, b);
     if (rc != TPD_OK) return rc;
     std::memcpy(&hs.h_models[16 * (size_t)entity], model, 16 * sizeof(float));
     hipError_t e = hipMemcpy(hs.d_models + 16 * (size_t)entity, model, 16 * sizeof(float), hipMemcpyHostToDevice);
@@ -691,7 +695,7 @@ int tpd_batch_set_views(tpd_ctx* c, const tpd_view* views, uint32_t nviews) {
             return fail(TPD_ERR_INVALID, "tpd_batch_set_views: view size must be 1..16384");
     }
     hipSetDevice(c->device);
-    int rc = wait_pending(b);
+    int rc = finish_pending(c, b);
     if (rc != TPD_OK) return rc;
     bool same_layout = !b->layout_dirty && b->views.size() == nviews;
     for (uint32_t v = 0; same_layout && v < nviews; ++v)
@@ -718,7 +722,7 @@ int tpd_batch_set_views(tpd_ctx* c, const tpd_view* views, uint32_t nviews) {
 int tpd_batch_set_graph(tpd_ctx* c, int enabled) {
     if (!c) return fail(TPD_ERR_INVALID, "null context");
     BatchState* b = state(c);
-    int rc = wait_pending(b);
+    int rc = finish_pending(c, b);
     if (rc != TPD_OK) return rc;
     b->use_graph = enabled != 0;
     return TPD_OK;

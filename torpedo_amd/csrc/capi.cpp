@@ -247,21 +247,22 @@ int tpd_upload(tpd_ctx* c, const float* gaussians, uint32_t n, const uint32_t* e
 int tpd_bind_device_scene(tpd_ctx* c, const float* d_gaussians, uint32_t n, uint32_t sh_degree) {
     if (!check_ctx(c)) return TPD_ERR_INVALID;
     if (n > 0 && !d_gaussians) return fail(TPD_ERR_INVALID, "tpd_bind_device_scene: null pointer");
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "sync");
     if (c->own_gauss) release(c->d_gauss);
     c->own_gauss = false;
     c->d_gauss = const_cast<float*>(d_gaussians);
     c->n = n;
     c->sh_degree = std::min<uint32_t>(sh_degree, 3u);
-    if (c->n_ent != 1 || !c->d_models) {
-        release(c->d_ent_idx);
-        release(c->d_models);
-        c->n_ent = 1;
-        c->h_models = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        hipError_t e;
-        if ((e = hipMalloc(&c->d_models, sizeof(float) * 16)) != hipSuccess) return hip_fail(e, "alloc");
-        if ((e = hipMemcpy(c->d_models, c->h_models.data(), sizeof(float) * 16, hipMemcpyHostToDevice)) != hipSuccess)
-            return hip_fail(e, "upload models");
-    }
+    // A bound scene is one entity with the identity model: whatever entity indices and transforms an
+    // earlier upload left (sized for that scene's point count) are dropped.
+    release(c->d_ent_idx);
+    release(c->d_models);
+    c->n_ent = 1;
+    c->h_models = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    if ((e = hipMalloc(&c->d_models, sizeof(float) * 16)) != hipSuccess) return hip_fail(e, "alloc");
+    if ((e = hipMemcpy(c->d_models, c->h_models.data(), sizeof(float) * 16, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "upload models");
     c->have_scene = true;
     c->rendered = false;
     return ensure_scene_buffers(c);
