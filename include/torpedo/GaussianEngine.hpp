@@ -9,6 +9,7 @@
 
 #include <torpedo/Camera.hpp>
 #include <torpedo/GaussianGeometry.hpp>
+#include <torpedo/HeadlessRenderer.hpp>
 #include <torpedo/Scene.hpp>
 #include <torpedo/TransformHost.hpp>
 #include <tpd_hip.h>
@@ -55,7 +56,10 @@ public:
         cfg.keep_unsorted = keepUnsorted ? 1u : 0u;
         check(tpd_create(&_ctx, &cfg), "GaussianEngine");
     }
-    ~GaussianEngine() { tpd_destroy(_ctx); }
+    ~GaussianEngine() {
+        if (_renderer && _renderer->boundEngine() == this) _renderer->resetEngine();  // swap images on _ctx
+        tpd_destroy(_ctx);
+    }
     GaussianEngine(const GaussianEngine&) = delete;
     GaussianEngine& operator=(const GaussianEngine&) = delete;
 
@@ -137,6 +141,19 @@ public:
     // into a device surface of row pitch `pitch` bytes (0: 4 * width), ordered on `stream`.
     void draw(void* surface, uint64_t pitch = 0, void* stream = nullptr) const {
         check(tpd_present_rgba8(_ctx, surface, pitch, stream), "GaussianEngine::draw");
+    }
+    // draw(SwapImage) as the reference's render loop calls it: into a renderer's swap image.
+    void draw(const SwapImage& swapImage, void* stream = nullptr) const {
+        if (!swapImage || swapImage.width != _w || swapImage.height != _h)
+            throw std::runtime_error("GaussianEngine::draw: the swap image does not match the frame size");
+        check(tpd_present_rgba8(_ctx, swapImage.image, swapImage.pitch, stream), "GaussianEngine::draw");
+    }
+    // Context::bindEngine: the renderer whose swap images live on this engine's context (shared, so
+    // that engine and context may be destroyed in either order).
+    void attachRenderer(std::shared_ptr<HeadlessRenderer> renderer) noexcept {
+        if (_renderer && _renderer != renderer) _renderer->resetEngine();
+        _renderer = std::move(renderer);
+        if (_renderer) _renderer->setEngine(this, _ctx);
     }
     void savePNG(const std::string& path) const { check(tpd_save_png(_ctx, path.c_str()), "GaussianEngine::savePNG"); }
     [[nodiscard]] float* deviceImage() const {
@@ -232,6 +249,7 @@ private:
     }
 
     tpd_ctx* _ctx = nullptr;
+    std::shared_ptr<HeadlessRenderer> _renderer;
     uint32_t _w, _h;
     uint32_t _n = 0;
     std::unique_ptr<TransformHost> _transformHost;

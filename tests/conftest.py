@@ -36,3 +36,19 @@ def small_scene(n, seed, spread=1.0, log_scale_mean=-4.5):
     g = scenes.uniform_scene(n, seed=seed, log_scale_mean=log_scale_mean)
     g[:, 0:3] *= spread
     return g
+
+
+def report_fp64(name, img32, img64):
+    """States one image's error against the FP64 twin (printed; also appended as a JSON line to the
+    file named by TPD_FP64_LOG when that is set) and returns (max, mean, 99.9th percentile)."""
+    import json
+    err = np.abs(np.asarray(img32, np.float64) - np.asarray(img64, np.float64))
+    stats = (float(err.max()), float(err.mean()), float(np.quantile(err, 0.999)))
+    print(f"[fp64] {name}: image vs FP64 twin max-abs {stats[0]:.3g}, mean {stats[1]:.3g}, "
+          f"p99.9 {stats[2]:.3g}", flush=True)
+    path = os.environ.get("TPD_FP64_LOG")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"image": name, "shape": list(np.shape(img32)), "max_abs": stats[0],
+                                "mean_abs": stats[1], "p999_abs": stats[2]}) + "\n")
+    return stats

@@ -3,6 +3,7 @@
 // compile it, rasterFrame a PerspectiveCamera, draw it into a device surface and a PNG file,
 // transform an entity, render a batch.  Writes what it rendered into the directory argv[1];
 // tests/test_gpu_cpp_api.py checks it against the oracle.
+#include <torpedo/Context.hpp>
 #include <torpedo/GaussianEngine.hpp>
 
 #include <cstdint>
@@ -117,6 +118,38 @@ int main(int argc, char** argv) {
             msg = e.what();
         }
         std::ofstream(out + "/error.txt") << msg;
+        // Context (Context.h:15-56): a camera before any engine throws; an engine bound before the
+        // renderer takes its frame size from initRenderer; a second initRenderer or bindEngine
+        // answers nullptr; the engine outlives the context and still draws.
+        {
+            std::string order;
+            auto context = tpd::Context<tpd::HeadlessRenderer>::create();
+            try {
+                (void)context->createCamera<tpd::PerspectiveCamera>();
+                order += "camera-before-engine:no-throw ";
+            } catch (const std::runtime_error&) {
+                order += "camera-before-engine:throws ";
+            }
+            auto eng = context->bindEngine<tpd::GaussianEngine>();
+            auto* r = context->initRenderer(96, 64);
+            order += context->initRenderer(10, 10) == nullptr ? "second-init:null " : "second-init:renderer ";
+            order += context->bindEngine<tpd::GaussianEngine>() == nullptr ? "second-bind:null " : "second-bind:engine ";
+            auto cam3 = context->createCamera<tpd::PerspectiveCamera>();
+            cam3->lookAt({0.2f, -0.1f, -3.0f}, {0.f, 0.f, 0.f}, {0.f, -1.f, 0.f});
+            eng->compile(scene);
+            eng->rasterFrame(*cam3);
+            const auto swap = r->launchFrame();
+            eng->draw(swap);
+            r->submitFrame(swap.index);
+            const auto drawn = r->readFrame(swap.index);
+            dump(out + "/ctx_frame.rgba8", drawn);
+            dump(out + "/ctx_view.f32", cam3->getViewMatrixData(), 16);
+            dump(out + "/ctx_proj.f32", cam3->getProjectionData(), 16);
+            context.reset();  // the engine outlives its context
+            eng->rasterFrame(*cam3);
+            dump(out + "/ctx_after.rgba8", eng->readRGBA8());
+            std::ofstream(out + "/ctx_order.txt") << order;
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "engine_api: %s\n", e.what());
         return 1;

@@ -364,3 +364,61 @@ def test_batch_depths_outside_the_sort_depth_window(engines, orc):
             assert passes == want, (name, passes)
     finally:
         beng.setProfiling(False)
+
+
+def _read_grads(eng, bufs, n):
+    out = []
+    for b, k in zip(bufs, (3, 3, 4, 1, 48)):
+        h = np.empty((n, k), np.float32)
+        eng.download(h, b)
+        out.append(h)
+    return out
+
+
+def test_back_to_back_steps_after_an_overflow(engines):
+    """Two batchStep calls issued without a synchronisation between them, the first of which
+    overflows its pair buffers (the cameras moved closer under a kept layout): the second call must
+    first render the first step again in grown buffers, so the accumulated gradients equal those of
+    two synchronised steps."""
+    beng, _ = engines
+    beng.clearScenes()
+    g = small_scene(60_000, seed=81, log_scale_mean=-3.0)
+    n = g.shape[0]
+    sid = beng.addScene(g)
+    sizes = [(512, 512), (400, 300)]
+    far = [view_tuple(sid, (0.0, 0.0, -30.0 - 5 * i), w, h) for i, (w, h) in enumerate(sizes)]
+    near = [view_tuple(sid, (0.2 * i, 0.1, -2.6), w, h) for i, (w, h) in enumerate(sizes)]
+    rng = np.random.default_rng(8)
+    dl = np.concatenate([rng.normal(size=(h, w, 3)).astype(np.float32).reshape(-1) for w, h in sizes])
+    d_dl = beng.alloc(dl.nbytes)
+    beng.upload(d_dl, dl)
+    bufs = [beng.alloc(n * k * 4) for k in (3, 3, 4, 1, 48)]
+    try:
+        beng.setBatchGraph(True)
+        beng.setViews(far)
+        beng.batchStep(d_dl, [bufs])
+        beng.syncBatch()
+        p_far = beng.batchPairCount
+        beng.setViews(near)  # same sizes and scene: the layout and its pair capacity are kept
+        beng.batchStep(d_dl, [bufs], accumulate=False)
+        beng.batchStep(d_dl, [bufs], accumulate=True)  # no synchronisation in between
+        beng.syncBatch()
+        p_near = beng.batchPairCount
+        say(f"[batch] back-to-back steps: pairs far {p_far} -> near {p_near}")
+        # the far frame sized the pair buffers to (P + P/8) x 1.15 (at least 65536 x 1.15) pairs,
+        # so the first near step overflowed them
+        assert p_near > 1.3 * max(p_far, 65536), (p_far, p_near)
+        got = _read_grads(beng, bufs, n)
+        beng.batchStep(d_dl, [bufs], accumulate=False)
+        beng.syncBatch()
+        beng.batchStep(d_dl, [bufs], accumulate=True)
+        beng.syncBatch()
+        ref = _read_grads(beng, bufs, n)
+        for name, a, b in zip(("means", "log_scales", "quats", "opacity_logits", "sh"), got, ref):
+            e = _rel(a, b)
+            say(f"[batch] back-to-back {name}: rel err {e:.2e}")
+            assert np.abs(b).max() > 0 and e < GRAD_TOL, (name, e)
+    finally:
+        for b in bufs:
+            beng.free(b)
+        beng.free(d_dl)

@@ -15,6 +15,7 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 BIN = os.path.join(ROOT, "tests", "cpp", "engine_api")
+HELLO = os.path.join(ROOT, "tests", "cpp", "hello_gaussian")
 
 
 def load(d, name, dtype, shape=None):
@@ -73,3 +74,35 @@ def test_cpp_api_matches_oracle(orc, tmp_path):
 
     assert np.array_equal(load(d, "image_after_empty.f32", np.float32, (200, 320, 3)), img_t)
     assert "no such scene" in open(os.path.join(d, "error.txt")).read()
+
+    # Context<HeadlessRenderer>: construction order, misuse answers and the swap-image path
+    order = open(os.path.join(d, "ctx_order.txt")).read().split()
+    assert order == ["camera-before-engine:throws", "second-init:null", "second-bind:null"]
+    c3 = cam("ctx_view.f32", "ctx_proj.f32", 96, 64)
+    ref3 = orc.render(rec, c3, 3, ent_idx=idx, models=models)
+    assert np.array_equal(load(d, "ctx_frame.rgba8", np.uint8, (64, 96, 4)), ref3["rgba8"])
+    assert np.array_equal(load(d, "ctx_after.rgba8", np.uint8, (64, 96, 4)), ref3["rgba8"])
+
+
+def test_hello_gaussian_demo_matches_oracle(orc, tmp_path):
+    """demo/HelloGaussian/main.cpp of the reference, built against include/torpedo/ with only its
+    window and orbit-control lines replaced (tests/cpp/hello_gaussian.cpp): 4 frames of the loop,
+    the last swap image read back equal (RGBA8, bit for bit) to the oracle's frame of the same
+    records and camera (SH degree 0, as the demo's settings)."""
+    if not os.path.exists(HELLO):
+        pytest.fail(f"{HELLO} is not built (run __graft_entry__.build())")
+    print(f"[cpp-api] running {HELLO}", flush=True)
+    r = subprocess.run([HELLO, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    d = str(tmp_path)
+    rec = load(d, "records.f32", np.float32, (-1, 60))
+    assert rec.shape[0] == 8193
+    frames, last = load(d, "frames.u32", np.uint32)
+    assert frames == 4 and last == 1  # two swap images taken in turn
+    v = load(d, "view.f32", np.float32)
+    pv, fo = orc.camera_upload(v, load(d, "proj.f32", np.float32))
+    ref = orc.render(rec, orc.Camera(v, pv, fo, 1280, 720), 0)
+    got = load(d, "frame.rgba8", np.uint8, (720, 1280, 4))
+    assert np.array_equal(got, ref["rgba8"])
+    assert ref["total"] > 8192 and got[..., :3].any()

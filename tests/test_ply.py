@@ -147,3 +147,21 @@ def test_errors(tmp_path):
     open(p, "wb").write(b"not a ply\n")
     with pytest.raises(TpdError):
         from_model(p)
+
+
+@pytest.mark.parametrize("fmt", ["binary_little_endian", "ascii"])
+def test_vertex_count_beyond_the_data(tmp_path, fmt):
+    """A header may claim any count: 4e9 vertices over a few bytes of data is refused before any
+    buffer is sized by it, and a count above 2^32 - 1 too."""
+    from torpedo_amd._capi import TpdError
+    from torpedo_amd.engine import from_model
+    props = ["x", "y", "z", "opacity", "rot_0", "rot_1", "rot_2", "rot_3", "scale_0", "scale_1", "scale_2",
+             "f_dc_0", "f_dc_1", "f_dc_2"]
+    for count, code in ((4_000_000_000, "INVALID"), (5_000_000_000, "CAPACITY")):
+        head = f"ply\nformat {fmt} 1.0\nelement vertex {count}\n"
+        head += "".join(f"property float {p}\n" for p in props) + "end_header\n"
+        body = b"\0" * (4 * len(props) * 3) if fmt != "ascii" else b"0 " * len(props) * 3
+        p = tmp_path / f"huge_{count}.ply"
+        open(p, "wb").write(head.encode() + body)
+        with pytest.raises(TpdError, match=code):
+            from_model(p)

@@ -46,21 +46,25 @@ def command(out: str = LIB) -> list[str]:
             "-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + ["-o", out]
 
 
-CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "engine_api.cpp")
-CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "engine_api")
+CPP_TESTS = ["engine_api", "hello_gaussian"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
 
 
-def build_cpp_test() -> str:
-    """The C++ API test program (include/torpedo/*.hpp over libtpd_hip.so), linked in-tree."""
+def build_cpp_test() -> list[str]:
+    """The C++ API test programs (include/torpedo/*.hpp over libtpd_hip.so), linked in-tree."""
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
-           "-L", HERE, "-ltpd_hip", "-Wl,-rpath,$ORIGIN/../../torpedo_amd", "-o", CPP_TEST_BIN + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("g++ failed building the C++ API test")
-    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
-    return CPP_TEST_BIN
+    out = []
+    for name in CPP_TESTS:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = [cxx, "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+               src, "-L", HERE, "-ltpd_hip", "-Wl,-rpath,$ORIGIN/../../torpedo_amd", "-o", exe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building tests/cpp/{name}.cpp")
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 def build(verbose: bool = False) -> str:

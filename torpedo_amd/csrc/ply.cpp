@@ -20,6 +20,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
+#include <stdexcept>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -121,6 +123,7 @@ int load(const char* path, std::vector<float>& out, uint32_t& n_out) {
     std::ifstream f(path, std::ios::binary | std::ios::ate);
     if (!f.is_open()) return fail(TPD_ERR_INVALID, std::string("Failed to open file: ") + path);
     const std::streamsize size = f.tellg();
+    if (size < 0) return fail(TPD_ERR_INVALID, std::string("Failed to read file: ") + path);
     f.seekg(0);
     std::vector<char> buf((size_t)size);
     if (size > 0 && !f.read(buf.data(), size)) return fail(TPD_ERR_INVALID, std::string("Failed to read file: ") + path);
@@ -235,6 +238,15 @@ int load(const char* path, std::vector<float>& out, uint32_t& n_out) {
         }
         return true;
     };
+    // The counts come from the file: before sizing anything by them, check that the data after the
+    // header can hold that many vertex rows (each value takes at least its binary size, or one
+    // character and a separator in ASCII).
+    uint64_t min_row = 0;
+    for (const auto& p : vert.props)
+        min_row += fmt == Format::Ascii ? 2 : type_size(p.is_list ? p.count_type : p.type);
+    if (vert.count > 0 && (min_row == 0 || vert.count > (buf.size() - pos + 1) / min_row))
+        return fail(TPD_ERR_INVALID, std::string("PLY: truncated data (the vertex count exceeds what the file holds): ") +
+                                         path);
     const uint32_t n = (uint32_t)vert.count;
     out.assign((size_t)n * 60, 0.0f);
     for (size_t ei = 0; ei < elements.size(); ++ei) {
@@ -280,7 +292,14 @@ int tpd_load_ply(const char* path, float** records, uint32_t* n) {
     *n = 0;
     std::vector<float> out;
     uint32_t count = 0;
-    const int rc = load(path, out, count);
+    int rc;
+    try {
+        rc = load(path, out, count);
+    } catch (const std::bad_alloc&) {
+        return fail(TPD_ERR_CAPACITY, std::string("tpd_load_ply: out of host memory reading ") + path);
+    } catch (const std::exception& ex) {
+        return fail(TPD_ERR_INVALID, std::string("tpd_load_ply: ") + ex.what());
+    }
     if (rc != TPD_OK) return rc;
     float* p = static_cast<float*>(std::malloc(std::max<size_t>(out.size(), 1) * sizeof(float)));
     if (!p) return fail(TPD_ERR_CAPACITY, "tpd_load_ply: out of host memory");
