@@ -156,6 +156,13 @@ int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per
  * d_vals_alt are scratch of the same size; *in_alt is set to 1 when the result is there. */
 int tpd_sort_pairs(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_keys_alt, uint32_t* d_vals_alt,
                    uint32_t n, uint32_t key_bits, void* stream, int* in_alt);
+/* As tpd_sort_pairs, allocating nothing: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The sort workspace is a fixed 8,536,064-byte device array per GPU, held until the process
+ * ends; n above 4,194,304 pairs returns TPD_ERR_CAPACITY.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race and must not be issued. */
+int tpd_sort_pairs_noalloc(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_keys_alt,
+                           uint32_t* d_vals_alt, uint32_t n, uint32_t key_bits, void* stream, int* in_alt);
 
 /* ----- autotuning (the reference's shapes are compile-time constants, GaussianEngine.h:121-124) -----
  * Times every radix-pass block shape on the last rendered frame's pairs and keeps the fastest

@@ -23,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtpd_hip.so")
 
 SOURCES = ["k_forward.hip", "k_sort.hip", "k_backward.hip", "k_batch.hip", "render.cpp", "backward.cpp", "batch.cpp",
-           "tuner.cpp", "k_util.hip", "ply.cpp", "present.cpp", "capi.cpp"]
+           "tuner.cpp", "k_util.hip", "noalloc.hip", "ply.cpp", "present.cpp", "capi.cpp"]
 
 
 def hipcc() -> str:

@@ -138,6 +138,8 @@ int tpd_create(tpd_ctx** out, const tpd_config* cfg) {
     const int dev = cfg ? cfg->device : 0;
     if (dev < 0 || dev >= ndev) return fail(TPD_ERR_INVALID, "tpd_create: bad device ordinal");
     if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    int rc = noalloc_setup(dev);
+    if (rc != TPD_OK) return rc;
     auto* c = new tpd_ctx();
     c->device = dev;
     c->keep_unsorted = cfg ? cfg->keep_unsorted != 0 : false;

@@ -93,6 +93,7 @@ void backward_release(tpd_ctx* c);
 void batch_release(tpd_ctx* c);
 void batch_set_sort_shape(tpd_ctx* c, uint32_t shape);
 int batch_view_rgba8(tpd_ctx* c, uint32_t view, const uint8_t** d_rgba, uint32_t* width, uint32_t* height);
+int noalloc_setup(int device);  // device address of the allocation-free sort's scratch (noalloc.hip)
 
 // Grows a device allocation to hold at least `count` elements (contents are not preserved).
 template <typename T>

@@ -635,3 +635,15 @@ def sort_pairs_device(engine: GaussianEngine, keys_ptr: int, vals_ptr: int, keys
                                ctypes.c_void_p(keys_alt_ptr), ctypes.c_void_p(vals_alt_ptr), int(n), int(key_bits),
                                None if stream is None else ctypes.c_void_p(stream), ctypes.byref(in_alt)))
     return bool(in_alt.value)
+
+
+def sort_pairs_device_noalloc(engine: GaussianEngine, keys_ptr: int, vals_ptr: int, keys_alt_ptr: int,
+                              vals_alt_ptr: int, n: int, key_bits: int, stream: int | None = None) -> bool:
+    """tpd_sort_pairs_noalloc: as sort_pairs_device, with no device allocation (n <= 4,194,304; the
+    library's fixed per-GPU scratch, shared by every call on that GPU)."""
+    in_alt = ctypes.c_int()
+    check(lib().tpd_sort_pairs_noalloc(engine._ctx, ctypes.c_void_p(keys_ptr), ctypes.c_void_p(vals_ptr),
+                                       ctypes.c_void_p(keys_alt_ptr), ctypes.c_void_p(vals_alt_ptr), int(n),
+                                       int(key_bits), None if stream is None else ctypes.c_void_p(stream),
+                                       ctypes.byref(in_alt)))
+    return bool(in_alt.value)
