@@ -331,6 +331,20 @@ def time_multiscene(eng, steps, warmup):
             "views_per_s": round(len(views) / wall, 2)}
 
 
+def hbm_ceiling(eng):
+    """Achievable HBM bandwidth on this GPU: copy / hipMemcpyAsync / nontemporal copy / read-only /
+    write-only streams of 256 MiB .. 4 GiB, kernels at 2048 .. 32768 blocks of 256 threads (GB/s,
+    copies counting read + written bytes)."""
+    out = {}
+    for nbytes in (256 << 20, 1 << 30, 4 << 30):
+        for kind in eng.BANDWIDTH_KINDS:
+            for blocks in ((0,) if kind == "memcpy" else (2048, 8192, 32768)):
+                key = f"{kind}/{nbytes >> 20}MiB" + (f"/{blocks}blocks" if blocks else "")
+                out[key] = round(eng.measureBandwidth(kind, nbytes, 20, blocks), 1)
+                log(f"[bench] {key}: {out[key]} GB/s")
+    return {"hbm_ceiling_gbs": out, "peak_gbs": HBM_PEAK_GBS}
+
+
 def cpu_baseline(g, views, budget_s=12.0):
     """The oracle (multithreaded C restatement) on the first views of the same 32-view workload,
     until about budget_s seconds of CPU work; views per second."""
@@ -362,7 +376,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
-    ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene", "pipelined"],
+    ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene", "pipelined",
+                                       "hbm-ceiling"],
                     default=None,
                     help="run one leg only (for profiler runs)")
     args = ap.parse_args()
@@ -403,6 +418,10 @@ def main():
         return
     if args.only == "multiscene":
         print(json.dumps(time_multiscene(eng, args.steps, args.warmup)), flush=True)
+        eng.close()
+        return
+    if args.only == "hbm-ceiling":
+        print(json.dumps(hbm_ceiling(eng)), flush=True)
         eng.close()
         return
     if args.only == "pipelined":

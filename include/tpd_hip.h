@@ -150,6 +150,10 @@ int tpd_get_stage_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
 int tpd_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
 /* Time of one plain device-to-device copy of `bytes` (bandwidth calibration), averaged over iters. */
 int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per_copy);
+/* Achievable bandwidth of a stream of `bytes`: kind 0 copy kernel, 1 hipMemcpyAsync device to device,
+ * 2 nontemporal copy kernel, 3 read-only kernel, 4 write-only kernel; `blocks` is the kernels' grid
+ * (0: 8192 blocks of 256 threads).  *ms is the mean time per pass over iters passes. */
+int tpd_measure_bandwidth(tpd_ctx* ctx, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks, float* ms);
 
 /* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----
  * Sorts n (key, value) pairs by key (stable); key bits at and above key_bits must be zero.  d_keys_alt /

@@ -89,7 +89,7 @@ int main(int argc, char** argv) {
     }
     const auto image = renderer->readFrame(static_cast<uint32_t>(last));
     dump(out + "/frame.rgba8", image.data(), image.size());
-    dump(out + "/records.f32", reinterpret_cast<const float*>(records.data()), records.size() * 60);
+    dump(out + "/records.f32", records.data(), records.size());  // the 240-byte records, as bytes
     dump(out + "/view.f32", camera->getViewMatrixData(), 16);
     dump(out + "/proj.f32", camera->getProjectionData(), 16);
     const uint32_t info[2] = { static_cast<uint32_t>(frame), static_cast<uint32_t>(last) };

@@ -178,14 +178,12 @@ struct OnesweepLds {
     static constexpr uint32_t R = 1u << BITS;
     static constexpr uint32_t kWaves = BLOCK / kWave;
     static constexpr uint32_t kTileKeys = BLOCK * ITEMS;
-    using Digit = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
     uint16_t whist[kWaves][R];  // per wave digit counters (then wave offsets), zero on entry
     uint32_t bstart[R];
     uint32_t adj[R];
     uint32_t wtot[8];
     uint32_t part;
     KT keys[kTileKeys];         // the partition reordered by digit (then its values)
-    Digit dig[kTileKeys];
 };
 
 // Decoupled look-back of one digit: kLookback predecessor descriptors are loaded at once (before
@@ -283,17 +281,17 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     }
 
     // Rank within the wave: peers with the same digit found with BITS ballots; the lowest lane of
-    // each group advances the wave's running digit counter.
+    // each group advances the wave's running digit counter.  Digits are recomputed from the keys
+    // wherever they are needed (a few instructions) rather than kept in registers or LDS.
+    const auto digit = [&](KT key) -> uint32_t {
+        if constexpr (MODE == kWide) return digit_of<BITS>(compact_key(key, a.lo_bits, a.lo_base), a.shift);
+        else return (key >> (a.shift - a.drop)) & ((1u << BITS) - 1u);
+    };
     const uint64_t lt = lanemask_lt();
-    uint32_t rank[ITEMS], dig[ITEMS];
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
-        if constexpr (MODE == kWide) dig[j] = digit_of<BITS>(compact_key(k[j], a.lo_bits, a.lo_base), a.shift);
-        else dig[j] = (k[j] >> (a.shift - a.drop)) & ((1u << BITS) - 1u);
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t d = dig[j];
+        const uint32_t d = digit(k[j]);
         uint64_t peers = ~0ull;
 #pragma unroll
         for (uint32_t b = 0; b < BITS; ++b) {
@@ -346,11 +344,10 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // Reorder the partition by digit in LDS (local position = digit start + wave offset + rank).
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
-        const uint32_t d = dig[j];
+        const uint32_t d = digit(k[j]);
         const uint32_t lp = L.bstart[d] + L.whist[wave][d] + rank[j];
         rank[j] = lp;
         L.keys[lp] = k[j];
-        L.dig[lp] = (typename Lds::Digit)d;
     }
 
     // Look-back per digit, then the global position adjustment of each digit run.
@@ -368,21 +365,26 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     }
     __syncthreads();
 
+    // Keys out, each to its digit's global start + its place in the partition; the global position
+    // of each output slot is kept for the values (the keys' LDS is reused for them).
     const uint32_t nvalid = min(kTileKeys, n - base);
-#pragma unroll 4
+    uint32_t gpos[ITEMS];
+#pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[L.adj[L.dig[i]] + i] = L.keys[i];
+        const KT key = L.keys[i];
+        gpos[t] = L.adj[digit(key)] + i;
+        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) s_vals[rank[j]] = v[j];
     __syncthreads();
-#pragma unroll 4
+#pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) a.vals_out[L.adj[L.dig[i]] + i] = s_vals[i];
+        if (full || i < nvalid) a.vals_out[gpos[t]] = s_vals[i];
     }
 }
 

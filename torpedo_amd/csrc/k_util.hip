@@ -1,5 +1,6 @@
-// Measurement helper: a plain streaming copy, used to calibrate HBM counters and to state the
-// achievable bandwidth beside the kernels' roofline fractions.
+// Measurement helpers: plain streaming kernels (copy, nontemporal copy, read-only, write-only) and
+// hipMemcpyAsync, used to calibrate HBM counters and to state the achievable bandwidth beside the
+// kernels' roofline fractions.
 #include "tpd_ctx.hpp"
 
 namespace tpd {
@@ -18,35 +19,96 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
+using f4v = float __attribute__((ext_vector_type(4)));
+
+// The same copy with nontemporal loads and stores (no L2 allocation for the streamed data).
+__global__ __launch_bounds__(256) void k_copy_nt(const f4v* __restrict__ src, f4v* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const f4v a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride);
+        const f4v c = __builtin_nontemporal_load(src + i + 2 * stride), d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+// Read-only stream: eight 16-byte loads in flight per thread, summed; the sum is stored only if it
+// equals a value it never takes, so the loads stay and nothing is written.
+__global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, float* __restrict__ sink, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0.0f;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+    }
+    for (; i < n; i += stride) acc += src[i].x;
+    if (acc == -1.2345e30f) sink[threadIdx.x] = acc;
+}
+
+// Write-only stream: 16-byte stores.
+__global__ __launch_bounds__(256) void k_fill(float4* __restrict__ dst, size_t n, float v) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const float4 f = make_float4(v, v, v, v);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = f;
+}
+
 }  // namespace tpd
 
 using namespace tpd;
 
-extern "C" int tpd_measure_copy(tpd_ctx* c, uint64_t bytes, uint32_t iters, float* ms_per_copy) {
-    if (!c || !ms_per_copy || bytes < 16 || iters == 0) return fail(TPD_ERR_INVALID, "tpd_measure_copy: bad argument");
+// kind: 0 copy kernel, 1 hipMemcpyAsync device to device, 2 nontemporal copy kernel, 3 read-only
+// kernel, 4 write-only kernel.  blocks: grid of the kernels (0: 8192).  *ms: mean time per pass.
+extern "C" int tpd_measure_bandwidth(tpd_ctx* c, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks,
+                                     float* ms) {
+    if (!c || !ms || bytes < 16 || iters == 0 || kind > 4)
+        return fail(TPD_ERR_INVALID, "tpd_measure_bandwidth: bad argument");
     hipSetDevice(c->device);
     const size_t n = bytes / 16;
+    const uint32_t grid = blocks ? blocks : 256 * 8 * 4;
     float4 *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipMalloc(&a, n * 16);
-    if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess && kind != 3 && kind != 4) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess && kind == 3) e = hipMalloc(&b, 256 * sizeof(float));
     if (e == hipSuccess) e = hipMemsetAsync(a, 1, n * 16, c->stream);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) {
-        const uint32_t grid = 256 * 8 * 4;
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);  // warm-up
-        hipEventRecord(e0, c->stream);
-        for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n);
-        hipEventRecord(e1, c->stream);
-        e = hipEventSynchronize(e1);
-        float ms = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        *ms_per_copy = ms / (float)iters;
-    }
+    auto one = [&]() -> hipError_t {
+        switch (kind) {
+            case 0: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, a, b, n); break;
+            case 1: return hipMemcpyAsync(b, a, n * 16, hipMemcpyDeviceToDevice, c->stream);
+            case 2:
+                hipLaunchKernelGGL(k_copy_nt, dim3(grid), dim3(256), 0, c->stream, reinterpret_cast<const f4v*>(a),
+                                   reinterpret_cast<f4v*>(b), n);
+                break;
+            case 3: hipLaunchKernelGGL(k_read, dim3(grid), dim3(256), 0, c->stream, a, reinterpret_cast<float*>(b), n); break;
+            default: hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, c->stream, a, n, 1.0f); break;
+        }
+        return hipGetLastError();
+    };
+    if (e == hipSuccess) e = one();  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = one();
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    *ms = t / (float)iters;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
-    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_measure_copy");
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_measure_bandwidth");
+}
+
+extern "C" int tpd_measure_copy(tpd_ctx* c, uint64_t bytes, uint32_t iters, float* ms_per_copy) {
+    return tpd_measure_bandwidth(c, 0, bytes, iters, 0, ms_per_copy);
 }

@@ -467,6 +467,17 @@ class GaussianEngine:
         check(lib().tpd_measure_copy(self._ctx, int(nbytes), int(iters), ctypes.byref(ms)))
         return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
+    BANDWIDTH_KINDS = ("copy", "memcpy", "copy_nontemporal", "read", "write")
+
+    def measureBandwidth(self, kind: str, nbytes: int, iters: int = 20, blocks: int = 0) -> float:
+        """GB/s of one streaming pattern over nbytes (tpd_measure_bandwidth); copies count the bytes
+        read and the bytes written."""
+        k = self.BANDWIDTH_KINDS.index(kind)
+        ms = ctypes.c_float()
+        check(lib().tpd_measure_bandwidth(self._ctx, k, int(nbytes), int(iters), int(blocks), ctypes.byref(ms)))
+        moved = 2.0 * nbytes if k <= 2 else float(nbytes)
+        return moved / (ms.value * 1e-3) / 1e9
+
     def deviceImagePtr(self) -> int:
         p = ctypes.c_void_p()
         check(lib().tpd_device_image(self._ctx, ctypes.byref(p)))
