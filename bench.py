@@ -34,17 +34,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "views/s batched vs plain; fwd+bwd ms @3M Gaussians 1080p; sort % HBM peak"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per pair per k_onesweep launch on this workload, per radix shape (mean over the frame's
-# passes: 64-bit, narrowing and 32-bit-key ones), from two rocprofv3 --pmc runs each:
+# HBM bytes per element per k_onesweep launch of the batched frame (32-bit keys and values; mean
+# over the frame's item and pair passes), per radix shape, from two rocprofv3 --pmc runs:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
-ONESWEEP_TRAFFIC_B_PER_PAIR = {4: 22.18, 6: 25.30}
-ONESWEEP_TRAFFIC_SOURCE = {
-    4: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 4 (48-bit keys): "
-        "1292.6 MB per launch (mean of 6 passes) at 58,288,090 pairs"),
-    6: ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --only multiview --sort-shape 6: "
-        "1474.7 MB per launch (mean of 5 passes) at 58,288,090 pairs "
-        "(profiles/batch_frame-multiview32-clusters1M.md)"),
-}
+ONESWEEP_TRAFFIC_B_PER_ELEMENT = {}
+ONESWEEP_TRAFFIC_SOURCE = {}
 VALU_PEAK_TFLOPS = 157.3
 
 
@@ -148,20 +142,22 @@ def time_multiview(eng, g, steps, warmup, tune=True):
         eng.syncBatch()
         gpu.append(eng.batchTimes()["frame"])
     eng.setBatchGraph(False)
-    st = []
+    st, so = [], []
     for _ in range(3):
         eng.rasterBatch()
         eng.syncBatch()
         st.append(eng.batchTimes())
+        so.append(eng.batchSortTimes())
     eng.setBatchGraph(True)
     eng.setProfiling(False)
     stages = {k: round(med([s[k] for s in st]), 4) for k in st[0]}
+    sorts = {k: (round(med([s[k] for s in so]), 4) if k.endswith("_ms") else so[0][k]) for k in so[0]}
     return {"views": len(views), "batched_views_per_s": round(len(views) / t_batch, 2),
             "plain_views_per_s": round(len(views) / t_plain, 2),
             "batched_over_plain": round(t_plain / t_batch, 3),
             "batched_ms_per_frame_wall": round(t_batch * 1000.0, 4), "batched_ms_gpu": round(med(gpu), 4),
             "plain_ms_32_views_wall": round(t_plain * 1000.0, 4), "pairs": P, "plain_pairs_sum": plain_pairs,
-            "eager_stage_ms": stages, "autotune": tuned, "autotune_plain": plain_tuned,
+            "eager_stage_ms": stages, "sorts": sorts, "autotune": tuned, "autotune_plain": plain_tuned,
             "resolutions": [[v[4], v[5]] for v in views]}
 
 
@@ -452,21 +448,34 @@ def main():
     del g2
     log(f"[bench] sort-load: P={P2}, stages {st2}")
 
-    # Roofline of the radix pass (k_onesweep) in the batched frame: 24 algorithmic bytes per pair
-    # per pass (key + value read and written); average launch = radix-pass time / passes.
-    es = mv["eager_stage_ms"]
-    ach, t_launch = onesweep_roofline(mv["pairs"], es["sort_passes"], es["radix_passes"])
+    # Roofline of the radix passes (k_onesweep, 32-bit keys) of the batched frame: the items are
+    # sorted by depth, then the pairs by tile; 16 algorithmic bytes per element per pass (4-B key
+    # and 4-B value, read and written), achieved = all passes' bytes / all passes' time (the
+    # time-weighted mean over the frame's launches).
+    es, so = mv["eager_stage_ms"], mv["sorts"]
     shape_b = (mv.get("autotune") or {}).get("best_shape", args.sort_shape)
-    traffic_b = ONESWEEP_TRAFFIC_B_PER_PAIR.get(shape_b)
+    pass_bytes = 16.0 * (so["items"] * so["item_passes"] + so["pairs"] * so["pair_passes"])
+    t_passes = (so["item_passes_ms"] + so["pair_passes_ms"]) / 1000.0
+    launches = so["item_passes"] + so["pair_passes"]
+    ach = pass_bytes / t_passes / 1e9 if t_passes > 0 else 0.0
+    traffic_b = ONESWEEP_TRAFFIC_B_PER_ELEMENT.get(shape_b)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": round(traffic_b * mv["pairs"]) if traffic_b else None,
-                "traffic_source": ONESWEEP_TRAFFIC_SOURCE.get(shape_b), "kernel": "k_onesweep",
-                "sort_shape": shape_desc(shape_b),
-                "pairs": mv["pairs"], "passes": es["radix_passes"], "avg_launch_us": round(t_launch * 1e6, 2),
-                "sort_ms": round(es["sort_histogram"] + es["sort_passes"], 4),
-                "sort_gkeys_per_s": round(sort_keys_per_s(mv["pairs"], es) / 1e9, 3),
+                "traffic": (round(traffic_b * pass_bytes / 16.0 / launches) if traffic_b else None),
+                "traffic_source": ONESWEEP_TRAFFIC_SOURCE.get(shape_b), "kernel": "k_onesweep (32-bit keys)",
+                "sort_shape": shape_desc(shape_b), "items": so["items"], "item_passes": so["item_passes"],
+                "pairs": so["pairs"], "pair_passes": so["pair_passes"],
+                "algorithmic_bytes_per_launch": round(pass_bytes / max(launches, 1)),
+                "avg_launch_us": round(t_passes / max(launches, 1) * 1e6, 2),
                 "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
+    # The sort metric as the benchmark defines it: 24 B x P x ceil(K / d) over the time to order the
+    # pairs by (tile, depth): both sorts, histograms included (K = global tile bits + 27 depth bits).
+    t_sort = (es["item_sort"] + es["pair_sort"]) / 1000.0
+    K = int(np.ceil(np.log2(max(sum(((v[0] + 15) // 16) * ((v[1] + 15) // 16) for v in mv["resolutions"]), 2)))) + 27
+    d = SORT_SHAPES[shape_b][0] if shape_b is not None and 0 <= shape_b < len(SORT_SHAPES) else 9
+    sort_metric = 24.0 * mv["pairs"] * ((K + d - 1) // d) / t_sort / 1e9 if t_sort > 0 else 0.0
+    roofline["sort_ms"] = round(t_sort * 1000.0, 4)
+    roofline["sort_gkeys_per_s"] = round(mv["pairs"] / t_sort / 1e9, 3) if t_sort > 0 else 0.0
     ach2, t2 = onesweep_roofline(P2, st2["sort_passes"], st2["radix_passes"])
     ach1, _ = onesweep_roofline(P, st["sort_passes"], st["radix_passes"])
 
@@ -512,10 +521,11 @@ def main():
                         "stage_ms": {k: round(v, 4) for k, v in st.items()
                                      if k not in ("radix_passes", "autotune", "blend_evaluations")},
                         "autotune": st.get("autotune")},
-        "sort_hbm_frac": {"multiview_32": round(ach / HBM_PEAK_GBS, 4), "pairs_multiview_32": mv["pairs"],
+        "sort_hbm_frac": {"multiview_32": round(sort_metric / HBM_PEAK_GBS, 4), "pairs_multiview_32": mv["pairs"],
+                          "multiview_32_key_bits": K,
                           "sort_load_variant": round(ach2 / HBM_PEAK_GBS, 4), "pairs_sort_load": P2,
                           "plain_1080p": round(ach1 / HBM_PEAK_GBS, 4), "pairs_plain_1080p": P,
-                          "gkeys_per_s": {"multiview_32": round(sort_keys_per_s(mv["pairs"], es) / 1e9, 3),
+                          "gkeys_per_s": {"multiview_32": roofline["sort_gkeys_per_s"],
                                           "sort_load_variant": round(sort_keys_per_s(P2, st2) / 1e9, 3),
                                           "plain_1080p": round(sort_keys_per_s(P, st) / 1e9, 3)}},
         "roofline": roofline,

@@ -247,9 +247,14 @@ int tpd_batch_device_images(tpd_ctx* ctx, float** d_images, uint64_t* total_pixe
 /* The sorted global (key, value) pairs of the last batch frame; *P receives the count. */
 int tpd_batch_read_pairs(tpd_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t capacity, uint32_t* P);
 /* With tpd_set_profiling on: ms[0] = GPU time of the last batch frame; for forward frames issued
- * without the graph also ms[1..6] = count pass, preprocess + duplicate, sort histograms, radix
- * passes, ranges, blend (else -1).  *passes receives the radix passes per frame. */
+ * without the graph also ms[1..6] = items (projection of every view), item sort, pair emission,
+ * pair sort, ranges + tile order, blend (else -1).  *passes receives the radix passes per frame
+ * (both sorts). */
 int tpd_batch_get_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
+/* The last eager profiled forward frame's two sorts (items by depth, then pairs by tile): ms[4] =
+ * item histogram, item radix passes, pair histogram, pair radix passes (milliseconds); counts[4]
+ * = items, item passes, pairs, pair passes. */
+int tpd_batch_get_sort_times(tpd_ctx* ctx, float* ms, uint32_t* counts);
 /* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
 int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
 

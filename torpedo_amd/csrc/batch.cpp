@@ -4,7 +4,7 @@
 // view as the plain frame would, in one launch per stage.
 //
 // A frame's pair count is known only on the device, so the graph sizes the sort and the pair
-// buffers by a capacity; the count pass raises an overflow flag when the capacity is too small,
+// buffers by a capacity; the emission raises an overflow flag when the capacity is too small,
 // every later kernel then skips its work, and the validation step (tpd_batch_synchronize, or any
 // read) grows the buffers, re-captures and renders again.
 #include "tpd_ctx.hpp"
@@ -34,7 +34,7 @@ struct BatchState {
     std::vector<BatchSceneDev> h_scenes;
     std::vector<BatchViewDev> h_views;
     std::vector<uint32_t> h_scene_views, h_block_scene, h_tile_view;
-    uint32_t nblocks = 0, nentries = 0, nitems = 0, ntiles = 0, nmats = 0, max_ent = 1;
+    uint32_t nblocks = 0, nitems = 0, ntiles = 0, nmats = 0, max_ent = 1;
     uint64_t npix = 0;
 
     // device tables
@@ -44,8 +44,7 @@ struct BatchState {
     uint32_t* d_block_scene = nullptr; size_t cap_block_scene = 0;
     uint32_t* d_tile_view = nullptr; size_t cap_tile_view = 0;
     EntityMats* d_mats = nullptr; size_t cap_mats = 0;
-    uint32_t* d_bases = nullptr; size_t cap_bases = 0;
-    uint64_t* d_scan_desc = nullptr; size_t cap_scan_desc = 0;
+    uint64_t* d_scan_desc = nullptr; size_t cap_scan_desc = 0;  // the emission's look-back, per partition
     uint32_t* d_counters = nullptr;
     uint32_t* h_counters = nullptr;  // pinned, written by the frame's last copy
 
@@ -53,6 +52,11 @@ struct BatchState {
     Splat* d_splats = nullptr; size_t cap_splats = 0;
     uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
     uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
+    uint64_t* d_rects = nullptr; size_t cap_rects = 0;  // per item emission record
+    uint32_t* d_ikeys = nullptr; size_t cap_ikeys = 0;  // item sort: depth keys and items, ping-pong
+    uint32_t* d_ivals = nullptr; size_t cap_ivals = 0;
+    uint32_t* d_ikeys_alt = nullptr; size_t cap_ikeys_alt = 0;
+    uint32_t* d_ivals_alt = nullptr; size_t cap_ivals_alt = 0;
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
     uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
     float* d_image = nullptr; size_t cap_image = 0;
@@ -61,18 +65,19 @@ struct BatchState {
     int32_t* d_final_last = nullptr; size_t cap_final_last = 0;
     uint8_t* d_rgba8 = nullptr; size_t cap_rgba8 = 0;
 
-    // pairs
-    uint64_t* d_keys = nullptr; size_t cap_keys = 0;
+    // pairs: global tile keys and item values, ping-pong
+    uint32_t* d_keys = nullptr; size_t cap_keys = 0;
     uint32_t* d_vals = nullptr; size_t cap_vals = 0;
-    uint64_t* d_keys_alt = nullptr; size_t cap_keys_alt = 0;
+    uint32_t* d_keys_alt = nullptr; size_t cap_keys_alt = 0;
     uint32_t* d_vals_alt = nullptr; size_t cap_vals_alt = 0;
+    uint64_t* d_full_keys = nullptr; size_t cap_full_keys = 0;  // 64-bit keys rebuilt for readers
     uint32_t cap = 0;  // pair capacity used by the recorded frame
-    SortWorkspace ws;
-    const uint64_t* sorted_keys = nullptr;   // 64-bit keys, or 32-bit ones when key_drop != kKeysWide
+    SortWorkspace ws;  // shared by the item sort and the pair sort (they run one after the other)
+    const uint32_t* sorted_items = nullptr;
+    const uint32_t* sorted_keys = nullptr;
     const uint32_t* sorted_vals = nullptr;
-    uint32_t key_drop = kKeysWide;
-    // Sort key of the recorded frame: the depth window unless a depth of this layout fell outside
-    // it (then the positive-float form until the layout changes).
+    // Depth key of the recorded frame's item sort: the depth window unless a depth of this layout
+    // fell outside it (then the positive-float form until the layout changes).
     bool depth_wide = false;
     uint32_t lo_bits() const { return depth_wide ? kDepthWideBits : kDepthWindowBits; }
     uint32_t lo_base() const { return depth_wide ? 0u : kDepthBase; }
@@ -97,9 +102,9 @@ struct BatchState {
     bool rendered = false;      // last frame validated
     bool kept_bwd_state = false;
     uint32_t last_pairs = 0;
-    hipEvent_t ev[9] = {};  // eager stage events [0..6], frame bracket [7], [8]
+    hipEvent_t ev[11] = {};  // eager stage events [0..6], frame bracket [7], [8], after each sort's histogram [9], [10]
     bool timed = false;
-    uint32_t passes = 0;    // radix passes of the recorded frame
+    uint32_t item_passes = 0, pair_passes = 0;  // radix passes of the recorded frame's two sorts
 };
 
 static void destroy_graphs(BatchState* b) {
@@ -120,10 +125,12 @@ void batch_release(tpd_ctx* c) {
         release(s.d_models);
     }
     release(b->d_scenes); release(b->d_views); release(b->d_scene_views); release(b->d_block_scene);
-    release(b->d_tile_view); release(b->d_mats); release(b->d_bases); release(b->d_scan_desc);
+    release(b->d_tile_view); release(b->d_mats); release(b->d_scan_desc);
     release(b->d_counters);
     if (b->h_counters) (void)hipHostFree(b->h_counters);
-    release(b->d_splats); release(b->d_tiles); release(b->d_offsets);
+    release(b->d_splats); release(b->d_tiles); release(b->d_offsets); release(b->d_rects);
+    release(b->d_ikeys); release(b->d_ivals); release(b->d_ikeys_alt); release(b->d_ivals_alt);
+    release(b->d_full_keys);
     release(b->d_ranges); release(b->d_tile_order); release(b->d_image); release(b->d_rgba8);
     release(b->d_final_T); release(b->d_final_last);
     release(b->d_keys); release(b->d_vals); release(b->d_keys_alt); release(b->d_vals_alt);
@@ -171,7 +178,7 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
     b->h_scenes.assign(ns, BatchSceneDev{});
     b->h_scene_views.clear();
     b->h_block_scene.clear();
-    uint64_t blocks = 0, entries = 0;
+    uint64_t blocks = 0;
     b->max_ent = 1;
     for (uint32_t s = 0; s < ns; ++s) {
         const HostScene& hs = b->scenes[s];
@@ -185,12 +192,10 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         d.first_block = (uint32_t)blocks;
         d.view_begin = (uint32_t)b->h_scene_views.size();
         d.nviews = (uint32_t)per_scene[s].size();
-        d.entry_base = (uint32_t)entries;
         const uint32_t nb = (hs.n + kPreBlock - 1) / kPreBlock;
         for (uint32_t i = 0; i < nb; ++i) b->h_block_scene.push_back(s);
         b->h_scene_views.insert(b->h_scene_views.end(), per_scene[s].begin(), per_scene[s].end());
         blocks += nb;
-        entries += (uint64_t)nb * d.nviews;
         b->max_ent = std::max(b->max_ent, std::max<uint32_t>(hs.n_ent, 1u));
     }
     b->h_views.assign(nv, BatchViewDev{});
@@ -219,11 +224,9 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         mats += std::max<uint32_t>(b->scenes[u.scene].n_ent, 1u);
         pix += (uint64_t)u.width * u.height;
     }
-    if (blocks > 0xFFFFFFFFull || entries > 0xFFFFFFFFull || items > 0xFFFFFFFFull || tiles >= (1ull << 31) ||
-        mats > 0xFFFFFFFFull)
-        return fail(TPD_ERR_CAPACITY, "tpd_batch: batch too large (items, tiles or blocks beyond 32 bits)");
+    if (blocks > 0xFFFFFFFFull || items > (uint64_t)kMaxSortKeys || tiles >= (1ull << 31) || mats > 0xFFFFFFFFull)
+        return fail(TPD_ERR_CAPACITY, "tpd_batch: batch too large (items beyond 2^30 - 1, tiles or blocks beyond 32 bits)");
     b->nblocks = (uint32_t)blocks;
-    b->nentries = (uint32_t)entries;
     b->nitems = (uint32_t)items;
     b->ntiles = (uint32_t)tiles;
     b->nmats = (uint32_t)mats;
@@ -237,9 +240,12 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_block_scene, b->cap_block_scene, std::max<size_t>(blocks, 1))) != hipSuccess ||
         (e = grow(b->d_tile_view, b->cap_tile_view, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_mats, b->cap_mats, std::max<size_t>(mats, 1))) != hipSuccess ||
-        (e = grow(b->d_bases, b->cap_bases, std::max<size_t>(entries, 1))) != hipSuccess ||
         (e = grow(b->d_splats, b->cap_splats, it)) != hipSuccess || (e = grow(b->d_tiles, b->cap_tiles, it)) != hipSuccess ||
         (e = grow(b->d_offsets, b->cap_offsets, it)) != hipSuccess ||
+        (e = grow(b->d_rects, b->cap_rects, it)) != hipSuccess ||
+        (e = grow(b->d_ikeys, b->cap_ikeys, it)) != hipSuccess || (e = grow(b->d_ivals, b->cap_ivals, it)) != hipSuccess ||
+        (e = grow(b->d_ikeys_alt, b->cap_ikeys_alt, it)) != hipSuccess ||
+        (e = grow(b->d_ivals_alt, b->cap_ivals_alt, it)) != hipSuccess ||
         (e = grow(b->d_ranges, b->cap_ranges, std::max<size_t>(tiles, 1))) != hipSuccess ||
         (e = grow(b->d_tile_order, b->cap_tile_order, std::max<size_t>(tile_order_words(tiles), 1))) != hipSuccess ||
         (e = grow(b->d_image, b->cap_image, std::max<size_t>(3 * pix, 1))) != hipSuccess ||
@@ -247,9 +253,9 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_final_last, b->cap_final_last, std::max<size_t>(pix, 1))) != hipSuccess ||
         (e = grow(b->d_rgba8, b->cap_rgba8, std::max<size_t>(4 * pix, 1))) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc");
-    if (!b->d_scan_desc || b->cap_scan_desc < std::max<size_t>(blocks, 1)) {
-        if ((e = grow(b->d_scan_desc, b->cap_scan_desc, std::max<size_t>(blocks, 1))) != hipSuccess)
-            return hip_fail(e, "tpd_batch: alloc");
+    const size_t nparts = std::max<size_t>((items + kEmitBlockItems - 1) / kEmitBlockItems, 1);
+    if (!b->d_scan_desc || b->cap_scan_desc < nparts) {
+        if ((e = grow(b->d_scan_desc, b->cap_scan_desc, nparts)) != hipSuccess) return hip_fail(e, "tpd_batch: alloc");
     }
     // look-back descriptors and counters start at zero; afterwards every frame leaves them so
     if ((e = hipMemset(b->d_scan_desc, 0, b->cap_scan_desc * sizeof(uint64_t))) != hipSuccess)
@@ -272,8 +278,8 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         e = hipMemcpy(b->d_block_scene, b->h_block_scene.data(), blocks * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "tpd_batch: upload layout");
     destroy_graphs(b);
-    b->cap = 0;  // the next frame counts its pairs first
-    b->depth_wide = false;  // and checks its depths
+    b->cap = 0;  // the next frame sizes its pair buffers first
+    b->depth_wide = false;  // and sorts on the depth window until a depth falls outside it
     b->rendered = false;
     b->layout_dirty = false;
     b->tables_dirty = true;
@@ -292,6 +298,28 @@ static int upload_tables(BatchState* b) {
     return TPD_OK;
 }
 
+// The sort workspace of both sorts: look-back descriptors for the larger of the pair capacity
+// and the item count.
+static int ensure_sort_ws(BatchState* b) {
+    hipError_t e;
+    if (!b->ws.partials) {
+        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMalloc(&b->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
+            (e = hipMalloc(&b->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
+            return hip_fail(e, "tpd_batch: alloc sort workspace");
+    }
+    const uint32_t parts = sort_parts(std::max(b->cap, b->nitems));
+    if (b->ws.parts_cap < parts) {
+        release(b->ws.desc);
+        if ((e = hipMalloc(&b->ws.desc, sizeof(uint32_t) * 2 * (size_t)parts * kMaxRadix)) != hipSuccess)
+            return hip_fail(e, "tpd_batch: alloc sort workspace");
+        b->ws.parts_cap = parts;
+        destroy_graphs(b);
+    }
+    return TPD_OK;
+}
+
 static int ensure_pairs(BatchState* b, size_t pairs) {
     pairs = std::max<size_t>(pairs, 1);
     if (b->cap_keys >= pairs && b->cap_vals >= pairs && b->cap_keys_alt >= pairs && b->cap_vals_alt >= pairs) {
@@ -303,7 +331,7 @@ static int ensure_pairs(BatchState* b, size_t pairs) {
             destroy_graphs(b);
             b->cap = cap;
         }
-        return TPD_OK;
+        return ensure_sort_ws(b);
     }
     const double h = 1.15;
     hipError_t e;
@@ -312,24 +340,9 @@ static int ensure_pairs(BatchState* b, size_t pairs) {
         (e = grow(b->d_keys_alt, b->cap_keys_alt, pairs, h)) != hipSuccess ||
         (e = grow(b->d_vals_alt, b->cap_vals_alt, pairs, h)) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc pairs");
-    const size_t cap = std::min({b->cap_keys, b->cap_vals, b->cap_keys_alt, b->cap_vals_alt, (size_t)kMaxSortKeys});
-    const uint32_t parts = sort_parts((uint32_t)cap);
-    if (!b->ws.partials) {
-        if ((e = hipMalloc(&b->ws.partials, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
-            (e = hipMemset(b->ws.partials, 0, sizeof(uint32_t) * kCountCopies * kMaxPasses * kMaxRadix)) != hipSuccess ||
-            (e = hipMalloc(&b->ws.offsets, sizeof(uint32_t) * kMaxPasses * kMaxRadix)) != hipSuccess ||
-            (e = hipMalloc(&b->ws.tickets, sizeof(uint32_t) * kMaxPasses)) != hipSuccess)
-            return hip_fail(e, "tpd_batch: alloc sort workspace");
-    }
-    if (b->ws.parts_cap < parts) {
-        release(b->ws.desc);
-        if ((e = hipMalloc(&b->ws.desc, sizeof(uint32_t) * 2 * (size_t)parts * kMaxRadix)) != hipSuccess)
-            return hip_fail(e, "tpd_batch: alloc sort workspace");
-        b->ws.parts_cap = parts;
-    }
     destroy_graphs(b);
-    b->cap = (uint32_t)cap;
-    return TPD_OK;
+    b->cap = (uint32_t)std::min({b->cap_keys, b->cap_vals, b->cap_keys_alt, b->cap_vals_alt, (size_t)kMaxSortKeys});
+    return ensure_sort_ws(b);
 }
 
 static int ensure_partials(BatchState* b, size_t pairs) {
@@ -348,14 +361,20 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     a.scene_views = b->d_scene_views;
     a.block_scene = b->d_block_scene;
     a.nblocks = b->nblocks;
+    a.nviews = (uint32_t)b->views.size();
+    a.nitems = b->nitems;
     a.mats = b->d_mats;
-    a.bases = b->d_bases;
     a.scan_desc = b->d_scan_desc;
     a.counters = b->d_counters;
     a.cap = b->cap;
     a.splats = b->d_splats;
     a.tiles = keep_bwd ? b->d_tiles : nullptr;
     a.offsets = keep_bwd ? b->d_offsets : nullptr;
+    a.rects = b->d_rects;
+    a.item_keys = b->d_ikeys;
+    a.item_vals = b->d_ivals;
+    a.lo_bits = b->lo_bits();
+    a.lo_base = b->lo_base();
     a.keys = b->d_keys;
     a.vals = b->d_vals;
     a.ranges = b->d_ranges;
@@ -363,9 +382,7 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     return a;
 }
 
-static uint32_t batch_key_bits(const BatchState* b) {
-    return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1) + b->lo_bits();  // compacted keys
-}
+static uint32_t tile_key_bits(const BatchState* b) { return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1); }
 
 static BatchBlendArgs blend_args(BatchState* b) {
     BatchBlendArgs a{};
@@ -387,25 +404,39 @@ static BatchBlendArgs blend_args(BatchState* b) {
 }
 
 // Issues the forward frame on s (no host synchronisation, no allocation, no memset: capturable).
-// With profiling on, stage events ev[0..5] are recorded (eager frames only).
+// With profiling on, stage events ev[0..6] are recorded (eager frames only): items, item sort,
+// emission, pair sort, ranges + tile order, blend.
 static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool events) {
     if (events) hipEventRecord(b->ev[0], s);
     launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, b->d_counters, s);
-    const BatchArgs a = forward_args(b, keep_bwd);
-    launch_batch_count(a, s);
+    BatchArgs a = forward_args(b, keep_bwd);
+    launch_batch_items(a, s);
     if (events) hipEventRecord(b->ev[1], s);
-    launch_batch_preprocess(a, s);
+    b->item_passes = b->nitems ? sort_shape_passes(b->ws.shape, b->lo_bits()) : 0;
+    bool items_alt = false;
+    if (b->nitems)
+        items_alt = launch_sort_pairs32(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
+                                        b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr);
+    else if (events)
+        hipEventRecord(b->ev[9], s);
+    b->sorted_items = items_alt ? b->d_ivals_alt : b->d_ivals;
     if (events) hipEventRecord(b->ev[2], s);
-    const uint32_t key_bits = batch_key_bits(b);
-    b->passes = sort_shape_passes(b->ws.shape, key_bits);
-    const bool in_alt = launch_sort_pairs(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
-                                          &b->d_counters[kCtrPairs], b->cap, key_bits, s, events ? b->ev[3] : nullptr,
-                                          b->lo_bits(), &b->key_drop, b->lo_base());
+    a.sorted_items = b->sorted_items;
+    launch_batch_emit(a, s);
+    if (events) hipEventRecord(b->ev[3], s);
+    const uint32_t key_bits = tile_key_bits(b);
+    b->pair_passes = sort_shape_passes(b->ws.shape, key_bits);
+    const bool in_alt = launch_sort_pairs32(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
+                                            &b->d_counters[kCtrPairs], b->cap, key_bits, s,
+                                            events ? b->ev[10] : nullptr);
     b->sorted_keys = in_alt ? b->d_keys_alt : b->d_keys;
     b->sorted_vals = in_alt ? b->d_vals_alt : b->d_vals;
     if (events) hipEventRecord(b->ev[4], s);
-    launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc, b->nblocks,
-                  &b->d_counters[kCtrScanTicket], s, b->cap, b->key_drop, b->lo_bits());
+    // the keys are global tiles (a shift of 0 past key_drop = lo_bits); lo_bits also tells the
+    // ranges kernel whether the items were sorted on the depth window
+    launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc,
+                  (b->nitems + kEmitBlockItems - 1) / kEmitBlockItems, &b->d_counters[kCtrScanTicket], s, b->cap, b->lo_bits(),
+                  b->lo_bits());
     launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);
     BatchBlendArgs ba = blend_args(b);
@@ -440,27 +471,11 @@ static void record_backward(BatchState* b, const float* dL, uint32_t accumulate,
     launch_batch_preprocess_bwd(p, s);
 }
 
-// First frame of a layout: count the pairs (mats + count pass), size the pair buffers exactly
-// (plus headroom), and put the count pass's state back to zero.
+// First frame of a layout: pair buffers for twice the items (or 2^20 pairs) to start with; a frame
+// with more pairs overflows them and is rendered again in grown buffers (validate).
 static int size_pairs(tpd_ctx* c, BatchState* b) {
-    hipStream_t s = c->stream;
-    launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, b->d_counters, s);
-    BatchArgs a = forward_args(b, false);
-    a.cap = 0;
-    launch_batch_count(a, s);
-    hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(uint32_t) * kCtrCount, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e == hipSuccess) e = hipMemsetAsync(b->d_scan_desc, 0, sizeof(uint64_t) * std::max<uint32_t>(b->nblocks, 1), s);
-    if (e == hipSuccess) e = hipMemsetAsync(b->d_counters, 0, sizeof(uint32_t) * kCtrCount, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(e, "tpd_batch: count pairs");
-    const uint32_t P = b->h_counters[kCtrPairs];
-    if (P > kMaxSortKeys) return fail(TPD_ERR_CAPACITY, "tpd_batch: more than 2^30 - 1 tile/Gaussian pairs");
-    if (b->h_counters[kCtrDepthOut] && !b->depth_wide) {
-        b->depth_wide = true;
-        destroy_graphs(b);
-    }
-    return ensure_pairs(b, std::max<size_t>((size_t)P + P / 8, 1u << 16));
+    (void)c;
+    return ensure_pairs(b, std::max<size_t>(2 * (size_t)b->nitems, 1u << 20));
 }
 
 static int prepare(tpd_ctx* c, BatchState* b) {
@@ -841,25 +856,24 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
     if (capacity < b->last_pairs) return fail(TPD_ERR_INVALID, "tpd_batch_read_pairs: capacity too small");
     hipError_t e = hipSuccess;
     if (keys && b->last_pairs) {
-        const uint64_t* k = b->sorted_keys;
-        if (b->key_drop != kKeysWide) {
-            // the sort left 32-bit keys: rebuild the 64-bit ones in the other (free) key buffer
-            uint64_t* full = b->sorted_keys == b->d_keys ? b->d_keys_alt : b->d_keys;
-            launch_full_keys(reinterpret_cast<const uint32_t*>(b->sorted_keys), b->key_drop, b->lo_bits(), b->sorted_vals,
-                             b->d_splats, b->last_pairs, full, c->stream);
+        // the sorted keys are global tiles: the 64-bit keys (tile << 32) | depth bits are rebuilt
+        // from the tile and the depth of each value's splat
+        e = grow(b->d_full_keys, b->cap_full_keys, b->last_pairs);
+        if (e == hipSuccess) {
+            launch_full_keys(b->sorted_keys, 0u, 0u, b->sorted_vals, b->d_splats, b->last_pairs, b->d_full_keys,
+                             c->stream);
             e = hipStreamSynchronize(c->stream);
-            k = full;
         }
-        if (e == hipSuccess) e = hipMemcpy(keys, k, sizeof(uint64_t) * b->last_pairs, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(keys, b->d_full_keys, sizeof(uint64_t) * b->last_pairs, hipMemcpyDeviceToHost);
     }
     if (e == hipSuccess && vals && b->last_pairs)
         e = hipMemcpy(vals, b->sorted_vals, sizeof(uint32_t) * b->last_pairs, hipMemcpyDeviceToHost);
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_read_pairs");
 }
 
-// ms[0] = whole frame (first launch to last); for eager profiled forward frames ms[1..6] = count
-// pass (with entity matrices), preprocess + duplicate, sort histograms, radix passes, ranges,
-// blend (else -1).  *passes = radix passes per frame.
+// ms[0] = whole frame (first launch to last); for eager profiled forward frames ms[1..6] = items
+// (with entity matrices), item sort, pair emission, pair sort, ranges + tile order, blend (else
+// -1).  *passes = radix passes per frame (item sort + pair sort).
 int tpd_batch_get_times(tpd_ctx* c, float* ms, uint32_t* passes) {
     if (!c || !ms) return fail(TPD_ERR_INVALID, "tpd_batch_get_times: bad argument");
     hipSetDevice(c->device);
@@ -870,12 +884,34 @@ int tpd_batch_get_times(tpd_ctx* c, float* ms, uint32_t* passes) {
     for (int i = 1; i <= 6; ++i) ms[i] = -1.0f;
     if (e == hipSuccess && !b->use_graph && b->pending_mode == 0)
         for (int i = 1; i <= 6 && e == hipSuccess; ++i) e = hipEventElapsedTime(&ms[i], b->ev[i - 1], b->ev[i]);
-    if (passes) *passes = b->passes;
+    if (passes) *passes = b->item_passes + b->pair_passes;
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_get_times");
 }
 
-// Times every radix-pass block shape on the batch's own pairs (full frames, eager) and keeps the
-// fastest for the batch's sort; outputs as tpd_autotune.
+// The last eager profiled forward frame's two sorts: ms = {item histogram, item radix passes, pair
+// histogram, pair radix passes}; counts = {items, item passes, pairs, pair passes}.
+int tpd_batch_get_sort_times(tpd_ctx* c, float* ms, uint32_t* counts) {
+    if (!c || !ms) return fail(TPD_ERR_INVALID, "tpd_batch_get_sort_times: bad argument");
+    hipSetDevice(c->device);
+    BatchState* b = state(c);
+    if (!c->profile || !b->timed || b->use_graph || b->pending_mode != 0)
+        return fail(TPD_ERR_STATE, "tpd_batch_get_sort_times: no eager profiled forward frame");
+    hipError_t e = hipEventSynchronize(b->ev[8]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], b->ev[1], b->ev[9]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], b->ev[9], b->ev[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], b->ev[3], b->ev[10]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[3], b->ev[10], b->ev[4]);
+    if (counts) {
+        counts[0] = b->nitems;
+        counts[1] = b->item_passes;
+        counts[2] = b->last_pairs;
+        counts[3] = b->pair_passes;
+    }
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_get_sort_times");
+}
+
+// Times every radix-pass block shape on the batch's own items and pairs (both sorts of full
+// eager frames) and keeps the fastest for the batch; outputs as tpd_autotune.
 int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best) {
     if (!c) return fail(TPD_ERR_INVALID, "null context");
     hipSetDevice(c->device);
@@ -895,12 +931,13 @@ int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* 
         for (int rep = 0; rep < 4 && rc == TPD_OK; ++rep) {
             if ((rc = launch_frame(c, b, 0, nullptr, 0, s)) != TPD_OK) break;
             if ((rc = validate(c, b)) != TPD_OK) break;
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, b->ev[2], b->ev[4]) != hipSuccess) {
+            float ms_items = 0.0f, ms_pairs = 0.0f;  // both sorts
+            if (hipEventElapsedTime(&ms_items, b->ev[1], b->ev[2]) != hipSuccess ||
+                hipEventElapsedTime(&ms_pairs, b->ev[3], b->ev[4]) != hipSuccess) {
                 rc = fail(TPD_ERR_HIP, "tpd_batch_autotune: timing");
                 break;
             }
-            if (rep > 0) t.push_back(ms);
+            if (rep > 0) t.push_back(ms_items + ms_pairs);
         }
         if (!t.empty()) {
             std::sort(t.begin(), t.end());

@@ -1,12 +1,12 @@
 // Backward pass of the forward frame.  The reference has no backward (torpedo/differential is
 // an empty stub, GaussianRasterizer.h:1-8); this differentiates the forward of project.slang and
 // blend.slang as restated in k_forward.hip:
-//   k_blend_bwd       per tile: re-runs the front-to-back blend to find each pixel's final
-//                     transmittance and last contributor (nothing is stored by the forward), then
-//                     walks the tile's splats back to front recovering T_i = T_{i+1} / (1 - a_i),
-//                     producing per splat dL/d(image x, y), dL/d(conic a, b, c), dL/d(rgb),
-//                     dL/d(opacity); wave sums are combined in LDS and stored once per
-//                     (tile, splat) pair at the pair's pre-sort position (blend_bwd_tile).
+//   k_blend_bwd       per tile: starts from each pixel's final transmittance and last
+//                     contributor (stored by the forward blend) and walks the tile's splats back
+//                     to front recovering T_i = T_{i+1} / (1 - a_i), producing per splat
+//                     dL/d(image x, y), dL/d(conic a, b, c), dL/d(rgb), dL/d(opacity); row sums
+//                     are combined in LDS and stored once per (tile, splat) pair at the pair's
+//                     pre-sort position (blend_bwd_tile).
 //   k_preprocess_bwd  per Gaussian: chain rule through the conic inverse, the EWA projection
 //                     T Sigma T^T, the Jacobian, R(q) S, the projection of the mean and the SH
 //                     colour (band-3 typo included), to means, log-scales, quaternions, opacity

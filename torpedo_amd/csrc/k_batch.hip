@@ -5,12 +5,14 @@
 // layout is described with BatchArgs).
 //
 //   k_batch_mats            entity matrices of every (view, entity)
-//   k_batch_count           per (preprocess block, view) pair counts; decoupled look-back over
-//                           the blocks turns them into pair positions; total P and the overflow
-//                           flag; clears the tile ranges
-//   k_batch_preprocess      each block reads its 256 Gaussian records once and, for every view of
-//                           its scene, projects them, writes the splats and emits the pairs
-//   (sort, ranges)          the plain path's kernels over the global keys
+//   k_batch_items           each block reads its 256 Gaussian records once and, for every view of
+//                           its scene, projects them and writes per item the splat, the tile
+//                           rectangle and a depth key; clears the tile ranges
+//   (item sort)             the items by depth key (32-bit radix passes)
+//   k_batch_emit            the pairs in depth order: key = global tile, value = item, positions
+//                           by a decoupled look-back scan over the sorted items; total P and the
+//                           overflow flag
+//   (pair sort, ranges)     the pairs by global tile alone, then the plain path's ranges
 //   k_batch_blend           one 16x16 tile of some view per block (blend_tile)
 //   k_batch_blend_bwd       blend_bwd_tile per tile
 //   k_batch_preprocess_bwd  per Gaussian, the chain rule summed over the views of its scene, into
@@ -47,235 +49,174 @@ __device__ __forceinline__ bool project_in_view(const float* rec, const EntityMa
                             vw.gy, sh_degree, p);
 }
 
-// ------------------------------------------------------------------ count pass
+// ------------------------------------------------------------------ items
 
-// The read-only tables come as separate __restrict__ parameters so that the compiler may fetch
-// them with scalar loads (uniform addresses, no aliasing stores).
-__global__ __launch_bounds__(kPreBlock) void k_batch_count(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
+// Per (Gaussian, view) item: project, write the splat record of a visible item, its emission
+// record (and tile count for the backward) and its depth key and index for the item sort.
+// Each block reads its 256 Gaussian records once, for all views of their scene; the threads do
+// not communicate, so no barrier is needed.  The block also clears its share of the tile ranges.
+__global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
                                                            const BatchViewDev* __restrict__ views,
                                                            const uint32_t* __restrict__ scene_views,
                                                            const uint32_t* __restrict__ block_scene,
                                                            const EntityMats* __restrict__ mats) {
-    constexpr uint32_t kWaves = kPreBlock / kWave;
-    __shared__ uint32_t s_part;
-    __shared__ uint32_t s_w[kWaves][kWave];  // wave sums of the current chunk of 64 views
-
-    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
-    if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
+    const uint32_t tid = threadIdx.x;
     for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
         a.ranges[t] = make_uint2(0u, 0u);
-    if (a.nblocks == 0 && blockIdx.x == 0 && tid == 0) {  // no Gaussians: the one block reports P = 0
-        a.counters[kCtrPairs] = 0;
-        a.counters[kCtrOverflow] = 0;
-    }
-    __syncthreads();
-    // The ticket is block-uniform: read it into a scalar register so that the scene and view
-    // tables below are fetched with scalar loads.
-    const uint32_t b = __builtin_amdgcn_readfirstlane(s_part);
-    if (b >= a.nblocks) return;  // tickets are reset every frame; true only without Gaussians
-    const BatchSceneDev& sc = scenes[block_scene[b]];
-    const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
-    const uint32_t g = lb * kPreBlock + tid;
-    const bool valid = g < sc.n;
-    const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
-    const uint32_t entry0 = sc.entry_base + lb * nviews;
-    // Only position, opacity, quaternion and scale decide the tile count; the colour part of the
-    // projection is unused here.
-    float rec[60];
-#pragma unroll
-    for (int q = 0; q < 60; ++q) rec[q] = 0.0f;
-    if (valid) {
-        const float4* r4 = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float4 t = r4[q];
-            rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
-        }
-    }
-
-    // Per view: the block's pair count, stored in bases[] until the look-back gives positions.
-    uint32_t my_total = 0;  // threads 0..63: sum of the counts they stored
-    bool depth_out = false;  // an emitted depth outside the sort's depth window
-    for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
-        const uint32_t kn = min(kWave, nviews - k0);
-        for (uint32_t kk = 0; kk < kn; ++kk) {
-            const BatchViewDev& vw = views[scene_views[sc.view_begin + k0 + kk]];
-            uint32_t t = 0;
-            if (valid) {
-                Projected p;
-                project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
-                t = p.tiles;
-                depth_out |= t != 0 && depth_outside_window(p.vz);
-            }
-            const uint32_t ws = wave_incl_scan(t);
-            if (lane == kWave - 1) s_w[wave][kk] = ws;
-        }
-        __syncthreads();
-        if (tid < kn) {
-            uint32_t c = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < kWaves; ++w) c += s_w[w][tid];
-            a.bases[entry0 + k0 + tid] = c;
-            my_total += c;
-        }
-        __syncthreads();
-    }
-
-    if (ballot(depth_out) != 0 && lane == 0) atomicOr(&a.counters[kCtrDepthOut], 1u);
-
-    if (wave == 0) {
-        const uint64_t total = wave_sum_u64(my_total);
-        if (lane == 0) store_relaxed(&a.scan_desc[b], (b == 0 ? kDescP : kDescA) | total);
-        uint64_t excl = 0;
-        if (b > 0) excl = scan_lookback(a.scan_desc, b);
-        if (lane == 0) {
-            if (b > 0) store_relaxed(&a.scan_desc[b], kDescP | (excl + total));
-            if (b == a.nblocks - 1) {
-                const uint64_t P = excl + total;
-                a.counters[kCtrPairs] = P > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)P;
-                a.counters[kCtrOverflow] = P > a.cap ? 1u : 0u;
-            }
-        }
-        // counts -> first pair position of each (block, view), views in the scene's order
-        uint64_t carry = excl;
-        for (uint32_t k0 = 0; k0 < nviews; k0 += kWave) {
-            const uint32_t k = k0 + lane;
-            const uint32_t c = k < nviews ? a.bases[entry0 + k] : 0u;  // stored by this same thread
-            const uint32_t incl = wave_incl_scan(c);
-            if (k < nviews) {
-                const uint64_t pos = carry + incl - c;
-                a.bases[entry0 + k] = pos > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)pos;
-            }
-            carry += __shfl(incl, kWave - 1, kWave);
-        }
-    }
-}
-
-void launch_batch_count(const BatchArgs& a, hipStream_t s) {
-    const uint32_t nb = a.nblocks ? a.nblocks : 1u;  // one block clears the ranges of an empty batch
-    hipLaunchKernelGGL(k_batch_count, dim3(nb), dim3(kPreBlock), 0, s, a, a.scenes, a.views, a.scene_views,
-                       a.block_scene, a.mats);
-}
-
-// ------------------------------------------------------------------ preprocess + duplicate
-
-__global__ __launch_bounds__(kPreBlock) void k_batch_preprocess(const BatchArgs a,
-                                                                const BatchSceneDev* __restrict__ scenes,
-                                                                const BatchViewDev* __restrict__ views,
-                                                                const uint32_t* __restrict__ scene_views,
-                                                                const uint32_t* __restrict__ block_scene,
-                                                                const EntityMats* __restrict__ mats) {
-    // Views are software-pipelined so that one barrier per view suffices: the segment between two
-    // barriers emits the large splats' pairs of view k-1, derives the pair positions of view k and
-    // writes its own pairs, and projects view k+1.  The LDS arrays are double-buffered by view
-    // parity: a segment reads view k-1's emission arrays and view k's wave sums while it writes
-    // view k's emission arrays and view k+1's wave sums.
-    __shared__ uint32_t s_wave_sum[2][kPreBlock / kWave], s_wave_big[2][kPreBlock / kWave];
-    __shared__ uint32_t s_big_incl[2][kPreBlock], s_start[2][kPreBlock];
-    __shared__ int32_t s_x0[2][kPreBlock], s_y0[2][kPreBlock], s_w[2][kPreBlock];
-    __shared__ uint32_t s_depth[2][kPreBlock];
-
-    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     const uint32_t b = blockIdx.x;
+    if (b >= a.nblocks) return;  // the one block of a batch without Gaussians only clears
     const BatchSceneDev& sc = scenes[block_scene[b]];
-    const uint32_t lb = b - sc.first_block, nviews = sc.nviews;
-    const uint32_t g = lb * kPreBlock + tid;
-    const bool valid = g < sc.n;
-    const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
-    const uint32_t entry0 = sc.entry_base + lb * nviews;
-    const uint32_t sh_degree = sc.sh_degree;
-    if (nviews == 0) return;
+    const uint32_t g = (b - sc.first_block) * kPreBlock + tid;
+    if (g >= sc.n) return;
+    const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
+    const uint32_t culled_key = a.lo_bits >= 32 ? 0xFFFFFFFFu : (1u << a.lo_bits) - 1u;
 
     // The record is read once and kept in registers for every view.
     float rec[60];
-    if (valid) {
-        const float4* r4 = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
+    const float4* r4 = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
 #pragma unroll
-        for (int q = 0; q < 15; ++q) {
-            const float4 t = r4[q];
-            rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 60; ++q) rec[q] = 0.0f;
+    for (int q = 0; q < 15; ++q) {
+        const float4 t = r4[q];
+        rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
     }
-
-    // This thread's Gaussian in the view last projected, and its wave-inclusive counts.
-    Projected p;
-    uint32_t incl_w = 0, big_w = 0;
-    // Projects view k, writes the splat record and tile count, and publishes the wave totals of
-    // the tile counts and of the counts above kOwnTiles.
-    const auto project_view = [&](uint32_t k) {
+    bool depth_out = false;  // a visible depth outside the sort's depth window
+    for (uint32_t k = 0; k < sc.nviews; ++k) {
         const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
-        p.tiles = 0;
-        if (valid) {
-            if (project_in_view(rec, mats[vw.mat_base + e], vw, sh_degree, p)) store_splat(a.splats + item, true, p);
-            if (a.tiles) a.tiles[item] = p.tiles;
+        Projected p;
+        const bool vis = project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
+        uint64_t rect = 0;
+        uint32_t key = culled_key;
+        if (vis) {
+            store_splat(a.splats + item, true, p);
+            rect = pack_emit_rect(vw.tile_base + (uint32_t)p.y0 * vw.gx + (uint32_t)p.x0, vw.gx,
+                                  (uint32_t)(p.x1 - p.x0), (uint32_t)(p.y1 - p.y0));
+            key = (as_uint(p.vz) - a.lo_base) & culled_key;
+            depth_out |= depth_outside_window(p.vz);
         }
-        const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
-        incl_w = wave_incl_scan(t);
-        big_w = wave_incl_scan(big);
-        if (lane == kWave - 1) {
-            s_wave_sum[k & 1u][wave] = incl_w;
-            s_wave_big[k & 1u][wave] = big_w;
-        }
-    };
-
-    project_view(0);
-    uint32_t big_total_prev = 0, base_prev = 0;
-    for (uint32_t k = 0; k < nviews; ++k) {
-        __syncthreads();
-        if (k > 0) {
-            const uint32_t q = (k - 1) & 1u;
-            const BatchViewDev& pv = views[scene_views[sc.view_begin + k - 1]];
-            emit_large_pairs(s_big_incl[q], s_start[q], s_x0[q], s_y0[q], s_w[q], s_depth[q], big_total_prev,
-                             base_prev, pv.gx, pv.tile_base, a.cap, a.keys, a.vals, pv.item_base + lb * kPreBlock);
-        }
-        {
-            const uint32_t q = k & 1u;
-            const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
-            const uint32_t t = p.tiles, big = t > kOwnTiles ? t : 0u;
-            uint32_t wave_off = 0, big_off = 0, big_total = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
-                wave_off += (w < wave) ? s_wave_sum[q][w] : 0u;
-                const uint32_t vb = s_wave_big[q][w];
-                big_off += (w < wave) ? vb : 0u;
-                big_total += vb;
-            }
-            const uint32_t excl = wave_off + incl_w - t;
-            const uint32_t base = a.bases[entry0 + k];
-            s_big_incl[q][tid] = big_off + big_w;
-            if (big) {
-                s_start[q][tid] = excl;
-                s_x0[q][tid] = p.x0;
-                s_y0[q][tid] = p.y0;
-                s_w[q][tid] = p.x1 - p.x0;
-                s_depth[q][tid] = as_uint(p.vz);
-            } else if (t) {
-                emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), (uint64_t)base + excl, vw.gx, vw.tile_base, a.cap,
-                               a.keys, a.vals, vw.item_base + g);
-            }
-            if (a.offsets && valid) a.offsets[vw.item_base + g] = base + excl;
-            big_total_prev = big_total;
-            base_prev = base;
-        }
-        if (k + 1 < nviews) project_view(k + 1);
+        a.rects[item] = rect;
+        a.item_keys[item] = key;
+        a.item_vals[item] = item;
+        if (a.tiles) a.tiles[item] = p.tiles;
     }
-    __syncthreads();
-    {
-        const uint32_t q = (nviews - 1) & 1u;
-        const BatchViewDev& pv = views[scene_views[sc.view_begin + nviews - 1]];
-        emit_large_pairs(s_big_incl[q], s_start[q], s_x0[q], s_y0[q], s_w[q], s_depth[q], big_total_prev, base_prev,
-                         pv.gx, pv.tile_base, a.cap, a.keys, a.vals, pv.item_base + lb * kPreBlock);
-    }
+    if (ballot(depth_out) != 0 && __lane_id() == 0) atomicOr(&a.counters[kCtrDepthOut], 1u);
 }
 
-void launch_batch_preprocess(const BatchArgs& a, hipStream_t s) {
-    if (a.nblocks == 0) return;
-    hipLaunchKernelGGL(k_batch_preprocess, dim3(a.nblocks), dim3(kPreBlock), 0, s, a, a.scenes, a.views,
-                       a.scene_views, a.block_scene, a.mats);
+void launch_batch_items(const BatchArgs& a, hipStream_t s) {
+    const uint32_t nb = a.nblocks ? a.nblocks : 1u;  // one block clears the ranges of an empty batch
+    hipLaunchKernelGGL(k_batch_items, dim3(nb), dim3(kPreBlock), 0, s, a, a.scenes, a.views, a.scene_views,
+                       a.block_scene, a.mats);
+}
+
+// ------------------------------------------------------------------ pair emission in depth order
+
+// Blocks take partitions of kEmitItems depth-sorted items (kEmitPerThread consecutive ones per
+// thread, so that a thread has that many independent record loads in flight) from an atomic
+// ticket; item i's pairs go to base + the exclusive scan of the tile counts over the sorted items
+// (decoupled look-back over the partitions, as the plain preprocess), key = global tile, value =
+// item.  Items with at most kOwnTiles tiles write their own pairs; the larger ones are spread over
+// the block (pair j of them belongs to the item whose inclusive count of large tiles first exceeds
+// j).  The last partition stores the pair total and the overflow flag.
+constexpr uint32_t kEmitPerThread = 4;
+constexpr uint32_t kEmitItems = kEmitBlockItems;
+static_assert(kEmitItems == kPreBlock * kEmitPerThread, "emission partition");
+
+__global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
+    __shared__ uint32_t s_big_incl[kEmitItems], s_start[kEmitItems], s_val[kEmitItems];
+    __shared__ uint64_t s_rect[kEmitItems];
+    __shared__ uint64_t s_base;
+
+    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
+    __syncthreads();
+    const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);
+    const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
+    if (part >= nparts) return;  // the grid has nparts blocks: never true in a consistent state
+    const uint32_t i0 = part * kEmitItems + tid * kEmitPerThread;
+
+    uint32_t item[kEmitPerThread];
+    uint64_t rect[kEmitPerThread];
+    if (i0 + kEmitPerThread <= a.nitems) {
+        const uint4 v = *reinterpret_cast<const uint4*>(a.sorted_items + i0);  // i0 is a multiple of 4
+        item[0] = v.x; item[1] = v.y; item[2] = v.z; item[3] = v.w;
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j) rect[j] = a.rects[item[j]];
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j) {
+            item[j] = i0 + j < a.nitems ? a.sorted_items[i0 + j] : 0u;
+            rect[j] = i0 + j < a.nitems ? a.rects[item[j]] : 0ull;
+        }
+    }
+    uint32_t t[kEmitPerThread], mine = 0, mine_big = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) {
+        t[j] = emit_rect_w(rect[j]) * emit_rect_h(rect[j]);
+        mine += t[j];
+        mine_big += t[j] > kOwnTiles ? t[j] : 0u;
+    }
+    const uint32_t incl_w = wave_incl_scan(mine), big_w = wave_incl_scan(mine_big);
+    if (lane == kWave - 1) {
+        s_wave_sum[wave] = incl_w;
+        s_wave_big[wave] = big_w;
+    }
+    __syncthreads();
+    uint32_t wave_off = 0, block_total = 0, big_off = 0, big_total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+        const uint32_t v = s_wave_sum[w], vb = s_wave_big[w];
+        wave_off += (w < wave) ? v : 0u;
+        block_total += v;
+        big_off += (w < wave) ? vb : 0u;
+        big_total += vb;
+    }
+    // this thread's items: their block-local starts, and their inclusive counts of large tiles
+    uint32_t start = wave_off + incl_w - mine, big_incl = big_off + big_w - mine_big;
+    uint32_t first[kEmitPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) {
+        const uint32_t o = tid * kEmitPerThread + j;
+        first[j] = start;
+        if (t[j] > kOwnTiles) {
+            big_incl += t[j];
+            s_start[o] = start;
+            s_rect[o] = rect[j];
+            s_val[o] = item[j];
+        }
+        s_big_incl[o] = big_incl;
+        start += t[j];
+    }
+    if (wave == 0) {
+        if (lane == 0) store_relaxed(&a.scan_desc[part], (part == 0 ? kDescP : kDescA) | (uint64_t)block_total);
+        uint64_t excl = 0;
+        if (part > 0) excl = scan_lookback(a.scan_desc, part);
+        if (lane == 0) {
+            if (part > 0) store_relaxed(&a.scan_desc[part], kDescP | (excl + block_total));
+            s_base = excl;
+            if (part == nparts - 1) {
+                const uint64_t total = excl + block_total;
+                a.counters[kCtrPairs] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
+                a.counters[kCtrOverflow] = total > a.cap ? 1u : 0u;
+            }
+        }
+    }
+    __syncthreads();
+    const uint64_t base = s_base;
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) {
+        if (!t[j]) continue;
+        if (a.offsets) a.offsets[item[j]] = (uint32_t)(base + first[j]);
+        if (t[j] <= kOwnTiles) emit_own_rect(t[j], rect[j], base + first[j], a.cap, a.keys, a.vals, item[j]);
+    }
+    emit_large_rects(s_big_incl, s_start, s_rect, s_val, kEmitItems, big_total, base, a.cap, a.keys, a.vals);
+}
+
+void launch_batch_emit(const BatchArgs& a, hipStream_t s) {
+    const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
+    if (nparts == 0) return;
+    hipLaunchKernelGGL(k_batch_emit, dim3(nparts), dim3(kPreBlock), 0, s, a);
 }
 
 // ------------------------------------------------------------------ blend, forward and backward
@@ -335,6 +276,10 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
     if (g >= sc.n) return;
     const float* rec = sc.gauss + (size_t)g * 60;
     const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
+    // The same FP32 operations as a plain backward per view accumulated into the gradient buffers:
+    // every log-scale, quaternion, opacity and SH entry receives one term per view, added to the
+    // running sum directly; a mean receives three terms per view, so they are summed on their own
+    // first and then added.
     float gm[3] = {0.f, 0.f, 0.f}, gls[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f}, gop = 0.f;
     float gsh[48];
 #pragma unroll
@@ -352,7 +297,10 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
 #pragma unroll
             for (int q = 0; q < 9; ++q) gr[q] += src[t * 9 + q];
         const EntityMats& m = mats[vw.mat_base + e];
-        accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, gm, gls, gq, gop, gsh);
+        float vm[3] = {0.f, 0.f, 0.f};
+        accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, vm, gls, gq, gop, gsh);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gm[c] += vm[c];
     }
     const BatchGradsDev& gd = grads[si];
     store_param_grads(g, a.accumulate != 0, gm, gls, gq, gop, gsh, gd.means, gd.log_scales, gd.quats,

@@ -22,8 +22,8 @@ constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 
-template <uint32_t BITS>
-__device__ __forceinline__ uint32_t digit_of(uint64_t k, uint32_t shift) {
+template <uint32_t BITS, typename KT>
+__device__ __forceinline__ uint32_t digit_of(KT k, uint32_t shift) {
     return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
 }
 
@@ -50,16 +50,21 @@ constexpr uint32_t kHistUnroll = 8;
 
 // The key of the lane below (lane 0 keeps its own) by a wave-wide DPP shift: register moves, no
 // LDS traffic (a shuffle per digit and pass goes through the LDS crossbar).
+constexpr int kWaveShr1 = 0x138;  // DPP control wave_shr:1
 __device__ __forceinline__ uint64_t key_of_lane_below(uint64_t k) {
-    constexpr int kWaveShr1 = 0x138;  // DPP control wave_shr:1
     const int lo = (int)(uint32_t)k, hi = (int)(uint32_t)(k >> 32);
     const uint32_t plo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, kWaveShr1, 0xF, 0xF, false);
     const uint32_t phi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, kWaveShr1, 0xF, 0xF, false);
     return ((uint64_t)phi << 32) | plo;
 }
+__device__ __forceinline__ uint32_t key_of_lane_below(uint32_t k) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, kWaveShr1, 0xF, 0xF, false);
+}
 
-template <uint32_t BITS>
-__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const uint32_t* count_ptr,
+// KT = uint64_t: (tile | depth) keys, histogrammed on their compacted form; KT = uint32_t: keys
+// sorted as they are (lo_bits and lo_base unused).
+template <uint32_t BITS, typename KT>
+__global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys, const uint32_t* count_ptr,
                                                    uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t lo_base,
                                                    uint32_t tile_keys,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
@@ -78,20 +83,22 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     constexpr uint32_t kChunk = 256 * kHistUnroll;
     const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
     for (uint32_t c0 = blockIdx.x * kChunk; c0 < n; c0 += gridDim.x * kChunk) {
-        uint64_t k[kHistUnroll];
+        KT k[kHistUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
-            k[u] = i < n ? keys[i] : 0ull;
+            k[u] = i < n ? keys[i] : KT(0);
         }
+        if constexpr (sizeof(KT) == 8) {
 #pragma unroll
-        for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits, lo_base);
+            for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits, lo_base);
+        }
 #pragma unroll
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
             const bool valid = i < n;
             const uint64_t vmask = ballot(valid);
-            const uint64_t below = key_of_lane_below(k[u]);
+            const KT below = key_of_lane_below(k[u]);
 #pragma unroll
             for (uint32_t p = 0; p < kMaxPasses; ++p) {
                 if (p >= passes) break;
@@ -411,12 +418,17 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits) {
     return (key_bits + b - 1) / b;
 }
 
-template <uint32_t BITS>
-static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const uint64_t* keys, const uint32_t* count_ptr,
+template <uint32_t BITS, typename KT>
+static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const KT* keys, const uint32_t* count_ptr,
                         uint32_t n_max, uint32_t lo_bits, uint32_t lo_base, uint32_t tile, SortWorkspace& ws) {
-    hipLaunchKernelGGL(k_sort_hist<BITS>, dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits, lo_base, tile,
-                       ws.partials, ws.desc, ws.tickets);
+    hipLaunchKernelGGL((k_sort_hist<BITS, KT>), dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits,
+                       lo_base, tile, ws.partials, ws.desc, ws.tickets);
     hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+}
+
+static uint32_t hist_blocks(uint32_t n_max) {
+    uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
+    return hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
 }
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
@@ -429,8 +441,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     const uint32_t parts = (n_max + tile - 1) / tile;
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
 
-    uint32_t hb = (n_max + 256 * kHistUnroll * 4 - 1) / (256 * kHistUnroll * 4);
-    hb = hb < 1 ? 1 : (hb > kHistBlocks ? kHistBlocks : hb);
+    const uint32_t hb = hist_blocks(n_max);
     if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
     else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
@@ -464,6 +475,42 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.ticket = ws.tickets + p;
         hipLaunchKernelGGL(sh.kernel[mode], dim3(parts), dim3(sh.block), 0, s, o);
         const uint64_t* tk = kout; kout = const_cast<uint64_t*>(kin); kin = tk;
+        const uint32_t* tv = vout; vout = const_cast<uint32_t*>(vin); vin = tv;
+    }
+    return (passes & 1u) != 0;
+}
+
+bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                         const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
+                         hipEvent_t after_hist) {
+    const SortShape& sh = kShapes[ws.shape % kNumShapes];
+    const uint32_t R = 1u << sh.bits;
+    const uint32_t tile = sh.block * sh.items;
+    const uint32_t passes = (key_bits + sh.bits - 1) / sh.bits;
+    const uint32_t parts = (n_max + tile - 1) / tile;
+    if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
+    const uint32_t hb = hist_blocks(n_max);
+    if (sh.bits == 8) launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, tile, ws);
+    else launch_hist<9>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, tile, ws);
+    if (after_hist) hipEventRecord(after_hist, s);
+    const uint32_t* kin = keys;
+    const uint32_t* vin = vals;
+    uint32_t* kout = keys_alt;
+    uint32_t* vout = vals_alt;
+    for (uint32_t p = 0; p < passes; ++p) {
+        OnesweepArgs o;
+        o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
+        o.count_ptr = count_ptr; o.n_imm = n_max;
+        o.shift = p * sh.bits;
+        o.lo_bits = 32u;
+        o.lo_base = 0u;
+        o.drop = 0u;
+        o.offsets = ws.offsets + p * R;
+        o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
+        o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * R;
+        o.ticket = ws.tickets + p;
+        hipLaunchKernelGGL(sh.kernel[kNarrowed], dim3(parts), dim3(sh.block), 0, s, o);
+        const uint32_t* tk = kout; kout = const_cast<uint32_t*>(kin); kin = tk;
         const uint32_t* tv = vout; vout = const_cast<uint32_t*>(vin); vin = tv;
     }
     return (passes & 1u) != 0;

@@ -40,7 +40,7 @@ struct BlendLds {
 struct BlendBwdLds {
     BlendLds s;
     uint32_t g[kTilePixels];                // splat index of each staged slot
-    float acc[kTilePixels][kSplatGrads];    // per staged splat: the tile's gradient sums
+    double acc[kTilePixels][kSplatGrads];   // per staged splat: the tile's gradient sums (see blend_bwd_tile)
     int32_t last;
 };
 
@@ -307,11 +307,16 @@ __device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t 
 // Backward of one tile.  The forward frame stored each pixel's final transmittance and last
 // contributor (blend_tile's final_T / final_last); the tile's splats are walked back to front
 // recovering T_i = T_{i+1} / (1 - a_i) and the colour accumulated behind each splat, producing per splat dL/d(image x, y), dL/d(conic a, b, c), dL/d(rgb), dL/d(opacity).
-// Wave sums are combined in LDS; each (tile, splat) pair's sums are stored once, at the pair's
-// pre-sort position offsets[g] + row-major rank of the tile in g's rectangle (keygen.slang:47-53
-// order), so no global atomics are needed and the per-Gaussian sum is deterministic.  Every pair
-// of the tile's range is written (zeros behind the last contributor), so the partials need no
-// clearing beforehand.
+// Each 16-lane row (one 4x4 sub-block) sums its lanes with DPP row operations in a fixed order;
+// the rows' sums for a splat (at most one per sub-block, 16 in all) are added in LDS with float64
+// atomics.  Those FP32 row sums add exactly in float64 unless their magnitudes span more than
+// 2^25, so the tile's total, rounded once to FP32, does not depend on the order in which the rows
+// arrive, and a frame's gradients repeat bit for bit from run to run (the exception would need
+// row sums 2^25 apart whose float64 rounding then also moves the FP32 result).  Each (tile,
+// splat) pair's sums are stored once, at the pair's pre-sort position offsets[g] + row-major rank
+// of the tile in g's rectangle, so no global atomics are needed, and the per-Gaussian sum over
+// its tiles runs in tile order.  Every pair of the tile's range is written (zeros behind the last
+// contributor), so the partials need no clearing beforehand.
 __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& tv, uint2 range,
                                                const uint32_t* __restrict__ vals, const Splat* __restrict__ splats,
                                                const float* __restrict__ dL_dimg, const uint32_t* __restrict__ offsets,
@@ -325,7 +330,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     const float fx = (float)pxl, fy = (float)pyl;
     BlendLds& s = L.s;
     if (tid == 0) L.last = -1;
-    for (uint32_t x = tid; x < kTilePixels * kG; x += kTilePixels) (&L.acc[0][0])[x] = 0.0f;
+    for (uint32_t x = tid; x < kTilePixels * kG; x += kTilePixels) (&L.acc[0][0])[x] = 0.0;
     const uint16_t* list = s.list[sub_block_of(wave, lane >> 4)];
 
     // The forward frame stored each pixel's final transmittance and last blended splat.
@@ -410,7 +415,7 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
 #pragma unroll
                 for (uint32_t q = 0; q < kG; ++q) {
                     const float sum = row_sum16(v[q]);
-                    if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], sum);
+                    if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], (double)sum);
                 }
             }
         }
@@ -419,8 +424,8 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             float out[kG];
 #pragma unroll
             for (uint32_t q = 0; q < kG; ++q) {
-                out[q] = L.acc[tid][q];
-                L.acc[tid][q] = 0.0f;
+                out[q] = (float)L.acc[tid][q];
+                L.acc[tid][q] = 0.0;
             }
             store_pair_partial(tv, L.g[tid], splats, offsets, partials, out);
         }

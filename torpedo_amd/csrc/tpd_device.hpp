@@ -636,4 +636,69 @@ __device__ __forceinline__ void emit_large_pairs(const uint32_t* s_big_incl, con
     }
 }
 
+// Pair emission of the batch, in depth order (k_batch_emit).  Each item carries one 64-bit
+// emission record: the global tile of its rectangle's first tile (31 bits: global tiles stay below
+// 2^31), its view's tile-grid width and its rectangle's width and height in tiles (11 bits each:
+// grids are at most 1024 tiles wide and high); zero for an item without tiles.  Its k-th pair
+// (row-major over the rectangle) has key tile0 + (k / w) gx + k % w and value the item; positions
+// at or past cap are dropped (the frame overflowed).
+__host__ __device__ inline uint64_t pack_emit_rect(uint32_t tile0, uint32_t gx, uint32_t w, uint32_t h) {
+    return (uint64_t)tile0 | ((uint64_t)gx << 31) | ((uint64_t)w << 42) | ((uint64_t)h << 53);
+}
+__host__ __device__ inline uint32_t emit_rect_tile0(uint64_t r) { return (uint32_t)(r & 0x7FFFFFFFull); }
+__host__ __device__ inline uint32_t emit_rect_gx(uint64_t r) { return (uint32_t)(r >> 31) & 0x7FFu; }
+__host__ __device__ inline uint32_t emit_rect_w(uint64_t r) { return (uint32_t)(r >> 42) & 0x7FFu; }
+__host__ __device__ inline uint32_t emit_rect_h(uint64_t r) { return (uint32_t)(r >> 53) & 0x7FFu; }
+
+__device__ __forceinline__ void emit_own_rect(uint32_t tiles, uint64_t rect, uint64_t pos, uint64_t cap,
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t val) {
+    const uint32_t w = emit_rect_w(rect), gx = emit_rect_gx(rect);
+    uint32_t row = emit_rect_tile0(rect), x = 0;
+    for (uint32_t k = 0; k < tiles; ++k) {
+        if (pos + k < cap) {
+            keys[pos + k] = row + x;
+            vals[pos + k] = val;
+        }
+        if (++x == w) {
+            x = 0;
+            row += gx;
+        }
+    }
+}
+
+// The large items of a block: s_big_incl[o] is the inclusive count of large tiles over the block's
+// items 0..o (n of them), s_start/s_rect/s_val the block-local first pair, record and value of a
+// large item o.
+__device__ __forceinline__ void emit_large_rects(const uint32_t* s_big_incl, const uint32_t* s_start,
+                                                 const uint64_t* s_rect, const uint32_t* s_val, uint32_t n,
+                                                 uint32_t big_total, uint64_t base, uint64_t cap,
+                                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t j = threadIdx.x; j < big_total; j += kPreBlock) {
+        uint32_t lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_big_incl[mid] > j) hi = mid; else lo = mid + 1;
+        }
+        const uint32_t owner = lo;
+        const uint32_t k = j - (owner ? s_big_incl[owner - 1] : 0u);
+        const uint64_t rect = s_rect[owner];
+        const uint32_t w = emit_rect_w(rect);
+        uint32_t ry, rx;
+        if (k < (1u << 22)) {  // k / w through a float reciprocal, corrected by one step
+            int32_t q = (int32_t)((float)k * (1.0f / (float)w));
+            int32_t r = (int32_t)k - q * (int32_t)w;
+            if (r < 0) { --q; r += (int32_t)w; }
+            if (r >= (int32_t)w) { ++q; r -= (int32_t)w; }
+            ry = (uint32_t)q; rx = (uint32_t)r;
+        } else {
+            ry = k / w; rx = k - ry * w;
+        }
+        const uint64_t gpos = base + s_start[owner] + k;
+        if (gpos < cap) {
+            keys[gpos] = emit_rect_tile0(rect) + ry * emit_rect_gx(rect) + rx;
+            vals[gpos] = s_val[owner];
+        }
+    }
+}
+
 }  // namespace tpd

@@ -131,9 +131,11 @@ void launch_tile_order(const uint2* ranges, uint32_t ntiles, const uint32_t* cou
 //   pixels  view v's image starts at pixel views[v].pix_base of the concatenated images;
 //   blocks  preprocess block b covers Gaussians [256 (b - first_block), +256) of scene
 //           block_scene[b] and projects them into every view of that scene, reading each
-//           Gaussian record once; the (block, view) partitions get their pair positions from the
-//           count pass (table `bases`, ordered block-major, then the scene's views in order, so
-//           each view's pairs keep Gaussian order).
+//           Gaussian record once;
+//   order   items are sorted by depth key (stable: equal depths keep item order, which within a
+//           view is Gaussian order), their pairs are emitted in that order, and the pairs are
+//           then sorted by global tile alone (stable): within each tile they stay in (depth,
+//           Gaussian) order, the order of the reference's (tile << 32 | depth) sort.
 
 struct BatchSceneDev {
     const float* gauss;          // N x 60 records
@@ -142,7 +144,6 @@ struct BatchSceneDev {
     uint32_t n, sh_degree, n_ent;
     uint32_t first_block;        // first preprocess block of the scene
     uint32_t view_begin, nviews; // the scene's views: scene_views[view_begin, view_begin + nviews)
-    uint32_t entry_base;         // (block b, k-th view) entry = entry_base + (b - first_block) * nviews + k
     uint32_t pad;
 };
 
@@ -159,18 +160,22 @@ struct BatchArgs {
     const BatchViewDev* views;
     const uint32_t* scene_views;  // view indices grouped by scene
     const uint32_t* block_scene;  // scene of each preprocess block
-    uint32_t nblocks;
+    uint32_t nblocks, nviews, nitems;
     const EntityMats* mats;
-    uint32_t* bases;              // per (block, view) entry: first pair position (count pass output)
-    uint64_t* scan_desc;          // nblocks look-back descriptors, zero on entry
+    uint64_t* scan_desc;          // ceil(nitems / 1024) look-back descriptors of the emission, zero on entry
     uint32_t* counters;           // kCtr* block
     uint32_t cap;                 // pair capacity
     Splat* splats;                // per item (written for visible items only)
     uint32_t* tiles;              // per item tile counts, or null (kept for the backward)
-    uint32_t* offsets;            // per item pre-sort pair offsets, or null
-    uint64_t* keys;
-    uint32_t* vals;
-    uint2* ranges;                // all views' tiles, cleared by the count pass
+    uint32_t* offsets;            // per item first pair position before the tile sort, or null (backward)
+    uint64_t* rects;              // per item emission record (pack_emit_rect); zero if culled
+    uint32_t* item_keys;          // per item depth key: (depth bits - lo_base) in lo_bits bits, all ones
+    uint32_t* item_vals;          //   within lo_bits when culled; and the item itself
+    uint32_t lo_bits, lo_base;
+    const uint32_t* sorted_items; // emission input: the items in depth-key order
+    uint32_t* keys;               // pairs: global tile
+    uint32_t* vals;               //        item
+    uint2* ranges;                // all views' tiles, cleared by the item kernel
     uint32_t ntiles;
 };
 
@@ -217,8 +222,9 @@ struct BatchPreBwdArgs {
 
 void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, uint32_t nviews, uint32_t max_ent,
                        EntityMats* mats, uint32_t* counters, hipStream_t s);
-void launch_batch_count(const BatchArgs& a, hipStream_t s);
-void launch_batch_preprocess(const BatchArgs& a, hipStream_t s);
+void launch_batch_items(const BatchArgs& a, hipStream_t s);
+void launch_batch_emit(const BatchArgs& a, hipStream_t s);
+constexpr uint32_t kEmitBlockItems = 1024;  // sorted items per emission partition (k_batch_emit)
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s);
@@ -265,5 +271,10 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
                        hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
                        uint32_t lo_base = 0);
+// The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
+// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.
+bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                         const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
+                         hipEvent_t after_hist = nullptr);
 
 }  // namespace tpd

@@ -620,7 +620,7 @@ class GaussianEngine:
         check(lib().tpd_batch_read_pairs(self._ctx, ptr(k), ptr(v), max(P, 1), ctypes.byref(n)))
         return k[:P], v[:P]
 
-    BATCH_STAGES = ("frame", "count", "preprocess_duplicate", "sort_histogram", "sort_passes", "ranges", "blend")
+    BATCH_STAGES = ("frame", "items", "item_sort", "emit", "pair_sort", "ranges", "blend")
 
     def batchTimes(self) -> dict:
         """GPU milliseconds of the last profiled batch frame (stages only for eager forward frames)."""
@@ -630,6 +630,15 @@ class GaussianEngine:
         out = {k: float(v) for k, v in zip(self.BATCH_STAGES, ms) if v >= 0}
         out["radix_passes"] = int(passes.value)
         return out
+
+    def batchSortTimes(self) -> dict:
+        """The last eager profiled forward batch frame's two sorts (tpd_batch_get_sort_times)."""
+        ms = np.zeros(4, np.float32)
+        n = np.zeros(4, np.uint32)
+        check(lib().tpd_batch_get_sort_times(self._ctx, ptr(ms), ptr(n)))
+        return {"item_hist_ms": float(ms[0]), "item_passes_ms": float(ms[1]), "pair_hist_ms": float(ms[2]),
+                "pair_passes_ms": float(ms[3]), "items": int(n[0]), "item_passes": int(n[1]), "pairs": int(n[2]),
+                "pair_passes": int(n[3])}
 
     def autotuneBatch(self) -> dict:
         ms = np.zeros(64, np.float32)
