@@ -253,7 +253,8 @@ static int build_layout(tpd_ctx* c, BatchState* b) {
         (e = grow(b->d_final_last, b->cap_final_last, std::max<size_t>(pix, 1))) != hipSuccess ||
         (e = grow(b->d_rgba8, b->cap_rgba8, std::max<size_t>(4 * pix, 1))) != hipSuccess)
         return hip_fail(e, "tpd_batch: alloc");
-    const size_t nparts = std::max<size_t>((items + kEmitBlockItems - 1) / kEmitBlockItems, 1);
+    // look-back descriptors: the emission's, then the item offsets' (frames kept for a backward)
+    const size_t nparts = 2 * std::max<size_t>((items + kEmitBlockItems - 1) / kEmitBlockItems, 1);
     if (!b->d_scan_desc || b->cap_scan_desc < nparts) {
         if ((e = grow(b->d_scan_desc, b->cap_scan_desc, nparts)) != hipSuccess) return hip_fail(e, "tpd_batch: alloc");
     }
@@ -411,6 +412,7 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     launch_batch_mats(b->d_scenes, b->d_views, (uint32_t)b->views.size(), b->max_ent, b->d_mats, b->d_counters, s);
     BatchArgs a = forward_args(b, keep_bwd);
     launch_batch_items(a, s);
+    launch_batch_item_offsets(a, s);  // frames kept for a backward only (a.offsets set)
     if (events) hipEventRecord(b->ev[1], s);
     b->item_passes = b->nitems ? sort_shape_passes(b->ws.shape, b->lo_bits()) : 0;
     bool items_alt = false;
@@ -435,7 +437,8 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     // the keys are global tiles (a shift of 0 past key_drop = lo_bits); lo_bits also tells the
     // ranges kernel whether the items were sorted on the depth window
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc,
-                  (b->nitems + kEmitBlockItems - 1) / kEmitBlockItems, &b->d_counters[kCtrScanTicket], s, b->cap, b->lo_bits(),
+                  2 * ((b->nitems + kEmitBlockItems - 1) / kEmitBlockItems), &b->d_counters[kCtrScanTicket], s, b->cap,
+                  b->lo_bits(),
                   b->lo_bits());
     launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);

@@ -117,9 +117,9 @@ void launch_batch_items(const BatchArgs& a, hipStream_t s) {
 // item.  Items with at most kOwnTiles tiles write their own pairs; the larger ones are spread over
 // the block (pair j of them belongs to the item whose inclusive count of large tiles first exceeds
 // j).  The last partition stores the pair total and the overflow flag.
-constexpr uint32_t kEmitPerThread = 4;
 constexpr uint32_t kEmitItems = kEmitBlockItems;
-static_assert(kEmitItems == kPreBlock * kEmitPerThread, "emission partition");
+constexpr uint32_t kEmitPerThread = kEmitItems / kPreBlock;
+static_assert(kEmitPerThread % 4 == 0, "items are loaded four at a time");
 
 __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     __shared__ uint32_t s_part;
@@ -139,8 +139,11 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     uint32_t item[kEmitPerThread];
     uint64_t rect[kEmitPerThread];
     if (i0 + kEmitPerThread <= a.nitems) {
-        const uint4 v = *reinterpret_cast<const uint4*>(a.sorted_items + i0);  // i0 is a multiple of 4
-        item[0] = v.x; item[1] = v.y; item[2] = v.z; item[3] = v.w;
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; j += 4) {  // i0 is a multiple of 4
+            const uint4 v = *reinterpret_cast<const uint4*>(a.sorted_items + i0 + j);
+            item[j] = v.x; item[j + 1] = v.y; item[j + 2] = v.z; item[j + 3] = v.w;
+        }
 #pragma unroll
         for (uint32_t j = 0; j < kEmitPerThread; ++j) rect[j] = a.rects[item[j]];
     } else {
@@ -205,12 +208,82 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     __syncthreads();
     const uint64_t base = s_base;
 #pragma unroll
-    for (uint32_t j = 0; j < kEmitPerThread; ++j) {
-        if (!t[j]) continue;
-        if (a.offsets) a.offsets[item[j]] = (uint32_t)(base + first[j]);
-        if (t[j] <= kOwnTiles) emit_own_rect(t[j], rect[j], base + first[j], a.cap, a.keys, a.vals, item[j]);
-    }
+    for (uint32_t j = 0; j < kEmitPerThread; ++j)
+        if (t[j] && t[j] <= kOwnTiles) emit_own_rect(t[j], rect[j], base + first[j], a.cap, a.keys, a.vals, item[j]);
     emit_large_rects(s_big_incl, s_start, s_rect, s_val, kEmitItems, big_total, base, a.cap, a.keys, a.vals);
+}
+
+// For frames kept for a backward: the exclusive scan of the items' tile counts in item order,
+// where the backward keeps each item's per-pair partial sums (consecutive Gaussians of a view then
+// have adjacent partials, which the parameter-gradient pass reads item by item).  kEmitPerThread
+// items per thread, decoupled look-back over the second half of the scan descriptors.
+__global__ __launch_bounds__(kPreBlock) void k_batch_item_offsets(const BatchArgs a) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
+    __shared__ uint64_t s_base;
+    const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket2], 1u);
+    __syncthreads();
+    const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);
+    const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
+    if (part >= nparts) return;
+    uint64_t* desc = a.scan_desc + nparts;
+    const uint32_t i0 = part * kEmitItems + tid * kEmitPerThread;
+    uint32_t t[kEmitPerThread], mine = 0;
+    if (i0 + kEmitPerThread <= a.nitems) {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; j += 4) {  // i0 is a multiple of 4
+            const uint4 v = *reinterpret_cast<const uint4*>(a.tiles + i0 + j);
+            t[j] = v.x; t[j + 1] = v.y; t[j + 2] = v.z; t[j + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j) t[j] = i0 + j < a.nitems ? a.tiles[i0 + j] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) mine += t[j];
+    const uint32_t incl_w = wave_incl_scan(mine);
+    if (lane == kWave - 1) s_wave_sum[wave] = incl_w;
+    __syncthreads();
+    uint32_t wave_off = 0, block_total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+        const uint32_t v = s_wave_sum[w];
+        wave_off += (w < wave) ? v : 0u;
+        block_total += v;
+    }
+    if (wave == 0) {
+        if (lane == 0) store_relaxed(&desc[part], (part == 0 ? kDescP : kDescA) | (uint64_t)block_total);
+        uint64_t excl = 0;
+        if (part > 0) excl = scan_lookback(desc, part);
+        if (lane == 0) {
+            if (part > 0) store_relaxed(&desc[part], kDescP | (excl + block_total));
+            s_base = excl;
+        }
+    }
+    __syncthreads();
+    uint32_t off = (uint32_t)(s_base + wave_off + incl_w - mine);
+    uint32_t o[kEmitPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) {
+        o[j] = off;
+        off += t[j];
+    }
+    if (i0 + kEmitPerThread <= a.nitems) {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; j += 4)
+            *reinterpret_cast<uint4*>(a.offsets + i0 + j) = make_uint4(o[j], o[j + 1], o[j + 2], o[j + 3]);
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j)
+            if (i0 + j < a.nitems) a.offsets[i0 + j] = o[j];
+    }
+}
+
+void launch_batch_item_offsets(const BatchArgs& a, hipStream_t s) {
+    const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
+    if (nparts == 0 || !a.offsets || !a.tiles) return;
+    hipLaunchKernelGGL(k_batch_item_offsets, dim3(nparts), dim3(kPreBlock), 0, s, a);
 }
 
 void launch_batch_emit(const BatchArgs& a, hipStream_t s) {

@@ -187,6 +187,7 @@ __global__ void k_ranges(const void* keys, uint32_t* __restrict__ counters, uint
     for (uint32_t d = gid; d < n_scan_desc; d += nthreads) scan_desc[d] = 0;
     if (gid == 0) {
         *ticket = 0;
+        counters[kCtrScanTicket2] = 0;
         if (lo_bits == kDepthWindowBits && counters[kCtrDepthOut]) counters[kCtrOverflow] = 1u;
     }
     uint32_t* r = reinterpret_cast<uint32_t*>(ranges);

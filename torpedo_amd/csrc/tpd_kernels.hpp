@@ -11,6 +11,7 @@ enum : uint32_t {
     kCtrPairs = 1,        // pair count P written by the last preprocess partition
     kCtrOverflow = 2,     // 1 when P exceeded the key/value capacity
     kCtrDepthOut = 3,     // non-zero when an emitted depth lies outside the sort's depth window
+    kCtrScanTicket2 = 4,  // partition ticket of a second scan in the frame (the batch's item offsets)
     kCtrCount = 16,
 };
 
@@ -162,12 +163,14 @@ struct BatchArgs {
     const uint32_t* block_scene;  // scene of each preprocess block
     uint32_t nblocks, nviews, nitems;
     const EntityMats* mats;
-    uint64_t* scan_desc;          // ceil(nitems / 1024) look-back descriptors of the emission, zero on entry
+    uint64_t* scan_desc;          // 2 x ceil(nitems / 1024) look-back descriptors, zero on entry: the
+                                  // emission's, then the item offsets
     uint32_t* counters;           // kCtr* block
     uint32_t cap;                 // pair capacity
     Splat* splats;                // per item (written for visible items only)
     uint32_t* tiles;              // per item tile counts, or null (kept for the backward)
-    uint32_t* offsets;            // per item first pair position before the tile sort, or null (backward)
+    uint32_t* offsets;            // per item exclusive scan of the tile counts in item order, or null:
+                                  // where the backward keeps the item's per-pair partial sums
     uint64_t* rects;              // per item emission record (pack_emit_rect); zero if culled
     uint32_t* item_keys;          // per item depth key: (depth bits - lo_base) in lo_bits bits, all ones
     uint32_t* item_vals;          //   within lo_bits when culled; and the item itself
@@ -224,6 +227,7 @@ void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, u
                        EntityMats* mats, uint32_t* counters, hipStream_t s);
 void launch_batch_items(const BatchArgs& a, hipStream_t s);
 void launch_batch_emit(const BatchArgs& a, hipStream_t s);
+void launch_batch_item_offsets(const BatchArgs& a, hipStream_t s);
 constexpr uint32_t kEmitBlockItems = 1024;  // sorted items per emission partition (k_batch_emit)
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s);
