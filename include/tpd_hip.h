@@ -255,6 +255,9 @@ int tpd_batch_get_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
  * item histogram, item radix passes, pair histogram, pair radix passes (milliseconds); counts[4]
  * = items, item passes, pairs, pair passes. */
 int tpd_batch_get_sort_times(tpd_ctx* ctx, float* ms, uint32_t* counts);
+/* tpd_get_blend_evaluations for the last batch frame: the pixel-splat pairs whose power its blend
+ * evaluated (counted by blending that frame once more with a counting kernel; same image). */
+int tpd_batch_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
 /* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
 int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
 

@@ -101,6 +101,73 @@ def test_batch_views_of_one_scene(engines, orc):
         beng.rasterBatch()
         beng.syncBatch()
         check_batch(beng, plain, [g], [3], views, f"one-scene graph={graph}", orc=orc if graph else None)
+    from test_gpu_sort import NUM_SHAPES
+    try:
+        for shape in range(NUM_SHAPES):  # both sorts of the batch frame under every radix block shape
+            beng.setSortShape(shape)
+            beng.rasterBatch()
+            beng.syncBatch()
+            check_batch(beng, plain, [g], [3], views, f"one-scene shape {shape}")
+    finally:
+        beng.setSortShape(0)
+
+
+def test_batch_emission_of_large_footprints(engines, orc):
+    """Pair emission stages the pairs of items with at most four tiles in LDS, chunk by chunk of a
+    block's output range, and writes them directly when the block's larger items push that range
+    past four chunks (8,192 pairs).  Small and large splats mixed in depth order exercise both
+    paths and chunk boundaries that cut through items; every view must equal its plain frame."""
+    beng, plain = engines
+    beng.clearScenes()
+    rng = np.random.default_rng(5)
+    parts = [small_scene(5000, seed=81), small_scene(1500, seed=82, log_scale_mean=-2.8),
+             small_scene(600, seed=83, log_scale_mean=-1.6)]
+    g = np.concatenate(parts)[rng.permutation(7100)].astype(np.float32)
+    sid = beng.addScene(g)
+    sizes = [(640, 480), (1280, 720), (200, 150)]
+    eyes = [(0.0, 0.0, -3.2), (2.0, 0.5, -2.5), (-2.5, 1.0, 1.5)]
+    views = [view_tuple(sid, eyes[i], w, h) for i, (w, h) in enumerate(sizes)]
+    beng.setViews(views)
+    beng.setBatchGraph(True)
+    beng.rasterBatch()
+    beng.syncBatch()
+    P = beng.batchPairCount
+    say(f"[batch] large footprints: {P} pairs over {g.shape[0] * len(views)} items")
+    blocks = -(-g.shape[0] * len(views) // 1024)  # emission blocks of 1,024 depth-sorted items
+    assert P > 8192 * blocks  # so that some block's range is longer than four chunks
+    check_batch(beng, plain, [g], [3], views, "large footprints", orc=orc)
+
+
+def test_batch_blend_evaluations_equal_the_plain_frames(engines):
+    """The batch blend's evaluation count (the FLOP count behind its roofline figure) equals the sum
+    of the plain frames' counts over the views, and counting leaves the batch images as they were."""
+    from torpedo_amd import scenes
+    beng, plain = engines
+    beng.clearScenes()
+    g = scenes.cluster_scene(30_000, seed=11)
+    sid = beng.addScene(g)
+    sizes = [(320, 240), (257, 131), (640, 360), (96, 512)]
+    eyes = scenes.fibonacci_sphere(len(sizes), 8.0)
+    views = [view_tuple(sid, eyes[i], w, h, up=(0.0, 0.0, 1.0)) for i, (w, h) in enumerate(sizes)]
+    beng.setViews(views)
+    beng.setBatchGraph(True)
+    beng.rasterBatch()
+    beng.syncBatch()
+    before = [beng.readBatchImage(i) for i in range(len(views))]
+    ev = beng.batchBlendEvaluations()
+    for i in range(len(views)):
+        assert same_bits(beng.readBatchImage(i), before[i]), i
+    plain.uploadRecords(g, 3)
+    plain.setProfiling(True)
+    try:
+        total = 0
+        for view in views:
+            plain_frame(plain, view)
+            total += plain.stageTimes()["blend_evaluations"]
+    finally:
+        plain.setProfiling(False)
+    say(f"[batch] blend evaluations: batch {ev}, plain frames {total}")
+    assert ev == total and ev > 0
 
 
 def test_batch_uneven_scenes(engines):

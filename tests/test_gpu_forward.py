@@ -132,8 +132,9 @@ def test_nearly_equal_depths_every_sort_shape(eng, orc):
         recs.append(gaussian_point([x, y * 0.6, float(rng.integers(0, 6)) * ulp], 0.3, (0, 0, 0, 1),
                                    (0.01, 0.01, 0.01), sh))
     g = np.stack(recs).astype(np.float32)
+    from test_gpu_sort import NUM_SHAPES
     try:
-        for shape in range(8):
+        for shape in range(NUM_SHAPES):
             eng.setSortShape(shape)
             compare_frame(eng, orc, g, cam, 3, f"equal-depth-runs shape {shape}")
     finally:

@@ -73,11 +73,12 @@ def test_multiview_32_clusters_1m(engines, orc):
     beng.addScene(g)
     beng.setViews(views)
     beng.setBatchGraph(True)
+    tuned = beng.autotuneBatch()  # the radix block shape the benchmark runs with
     beng.rasterBatch()
     beng.syncBatch()
     keys, vals = beng.readBatchPairs()
     P = beng.batchPairCount
-    say(f"[fullsize] multiview-32: P={P}")
+    say(f"[fullsize] multiview-32: P={P}, radix shape {tuned['best_shape']}")
     assert np.all(keys[1:] >= keys[:-1])
     plain.uploadRecords(g, 3)
     total = 0

@@ -69,7 +69,7 @@ def test_sort_empty_and_invalid(eng):
         gpu_sort(eng, np.zeros(4, np.uint64), np.zeros(4, np.uint32), 0)
 
 
-NUM_SHAPES = 8  # 8-bit digits x 6 block shapes, 9-bit digits x 2
+NUM_SHAPES = 9  # 8-bit digits x 6 block shapes, 9-bit digits x 3
 
 
 def test_shape_count(eng):

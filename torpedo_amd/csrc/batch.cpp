@@ -71,6 +71,7 @@ struct BatchState {
     uint32_t* d_keys_alt = nullptr; size_t cap_keys_alt = 0;
     uint32_t* d_vals_alt = nullptr; size_t cap_vals_alt = 0;
     uint64_t* d_full_keys = nullptr; size_t cap_full_keys = 0;  // 64-bit keys rebuilt for readers
+    uint64_t* d_eval_count = nullptr;  // tpd_batch_get_blend_evaluations
     uint32_t cap = 0;  // pair capacity used by the recorded frame
     SortWorkspace ws;  // shared by the item sort and the pair sort (they run one after the other)
     const uint32_t* sorted_items = nullptr;
@@ -130,7 +131,7 @@ void batch_release(tpd_ctx* c) {
     if (b->h_counters) (void)hipHostFree(b->h_counters);
     release(b->d_splats); release(b->d_tiles); release(b->d_offsets); release(b->d_rects);
     release(b->d_ikeys); release(b->d_ivals); release(b->d_ikeys_alt); release(b->d_ivals_alt);
-    release(b->d_full_keys);
+    release(b->d_full_keys); release(b->d_eval_count);
     release(b->d_ranges); release(b->d_tile_order); release(b->d_image); release(b->d_rgba8);
     release(b->d_final_T); release(b->d_final_last);
     release(b->d_keys); release(b->d_vals); release(b->d_keys_alt); release(b->d_vals_alt);
@@ -872,6 +873,30 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
     if (e == hipSuccess && vals && b->last_pairs)
         e = hipMemcpy(vals, b->sorted_vals, sizeof(uint32_t) * b->last_pairs, hipMemcpyDeviceToHost);
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_read_pairs");
+}
+
+// Blends the last frame again with the counting kernel (same tiles, splats and order; it writes the
+// same image, and not the backward's per-pixel state) and returns the pixel-splat pairs whose power
+// the blend evaluated: the FLOP count of the frame's blend.
+int tpd_batch_get_blend_evaluations(tpd_ctx* c, uint64_t* evaluations) {
+    if (!c || !evaluations) return fail(TPD_ERR_INVALID, "tpd_batch_get_blend_evaluations: bad argument");
+    hipSetDevice(c->device);
+    BatchState* b = state(c);
+    int rc = validate(c, b);
+    if (rc != TPD_OK) return rc;
+    if (!b->rendered) return fail(TPD_ERR_STATE, "tpd_batch_get_blend_evaluations: no batch frame rendered");
+    hipError_t e = hipSuccess;
+    if (!b->d_eval_count) e = hipMalloc(&b->d_eval_count, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_eval_count, 0, sizeof(uint64_t), c->stream);
+    if (e != hipSuccess) return hip_fail(e, "tpd_batch_get_blend_evaluations: counter");
+    BatchBlendArgs ba = blend_args(b);
+    ba.final_T = nullptr;
+    ba.final_last = nullptr;
+    ba.eval_count = b->d_eval_count;
+    launch_batch_blend(ba, c->stream);
+    if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(evaluations, b->d_eval_count, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_get_blend_evaluations");
 }
 
 // ms[0] = whole frame (first launch to last); for eager profiled forward frames ms[1..6] = items

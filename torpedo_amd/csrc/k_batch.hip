@@ -121,12 +121,30 @@ constexpr uint32_t kEmitItems = kEmitBlockItems;
 constexpr uint32_t kEmitPerThread = kEmitItems / kPreBlock;
 static_assert(kEmitPerThread % 4 == 0, "items are loaded four at a time");
 
+// Pairs of the block's small rectangles staged per chunk before they are written out: the large
+// items' tables (start, value, rectangle) are done with by then and lend their space.
+constexpr uint32_t kEmitStage = 2 * kEmitItems;
+constexpr uint32_t kEmitMaxChunks = 4;
+constexpr uint32_t kNoPair = 0xFFFFFFFFu;  // an empty staging slot (no item has this index)
+union EmitLds {
+    struct {
+        uint32_t start[kEmitItems], val[kEmitItems];
+        uint64_t rect[kEmitItems];
+    } big;
+    struct {
+        uint32_t key[kEmitStage], val[kEmitStage];
+    } stage;
+};
+
 __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
-    __shared__ uint32_t s_big_incl[kEmitItems], s_start[kEmitItems], s_val[kEmitItems];
-    __shared__ uint64_t s_rect[kEmitItems];
+    __shared__ uint32_t s_big_incl[kEmitItems];
+    __shared__ EmitLds L;
     __shared__ uint64_t s_base;
+    uint32_t* const s_start = L.big.start;
+    uint32_t* const s_val = L.big.val;
+    uint64_t* const s_rect = L.big.rect;
 
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
@@ -207,10 +225,35 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     }
     __syncthreads();
     const uint64_t base = s_base;
-#pragma unroll
-    for (uint32_t j = 0; j < kEmitPerThread; ++j)
-        if (t[j] && t[j] <= kOwnTiles) emit_own_rect(t[j], rect[j], base + first[j], a.cap, a.keys, a.vals, item[j]);
     emit_large_rects(s_big_incl, s_start, s_rect, s_val, kEmitItems, big_total, base, a.cap, a.keys, a.vals);
+    // The small rectangles' pairs (at most kOwnTiles per item, written by the item's thread) go
+    // through LDS, chunk by chunk of the block's output range, so that the stores to HBM are
+    // consecutive across lanes; slots of the large items' pairs stay empty and are skipped.  A
+    // block whose large items make its range longer than kEmitMaxChunks chunks writes them directly.
+    if (block_total > kEmitMaxChunks * kEmitStage) {
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j)
+            if (t[j] && t[j] <= kOwnTiles) emit_own_rect(t[j], rect[j], base + first[j], a.cap, a.keys, a.vals, item[j]);
+        return;
+    }
+    for (uint32_t c0 = 0; c0 < block_total; c0 += kEmitStage) {
+        const uint32_t n = min(block_total - c0, kEmitStage);
+        __syncthreads();  // the large items' tables, or the previous chunk, are done with
+        for (uint32_t p = tid; p < n; p += kPreBlock) L.stage.val[p] = kNoPair;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kEmitPerThread; ++j)
+            if (t[j] && t[j] <= kOwnTiles) stage_own_rect(t[j], rect[j], first[j], c0, n, L.stage.key, L.stage.val, item[j]);
+        __syncthreads();
+        for (uint32_t p = tid; p < n; p += kPreBlock) {
+            const uint32_t v = L.stage.val[p];
+            const uint64_t g = base + c0 + p;
+            if (v != kNoPair && g < a.cap) {
+                a.keys[g] = L.stage.key[p];
+                a.vals[g] = v;
+            }
+        }
+    }
 }
 
 // For frames kept for a backward: the exclusive scan of the items' tile counts in item order,
@@ -301,6 +344,9 @@ __device__ __forceinline__ TileView batch_tile_view(const BatchViewDev* __restri
     return TileView{local % vw.gx, local / vw.gx, vw.cam.width, vw.cam.height, vw.gx, vw.gy, vw.pix_base};
 }
 
+// COUNT: also adds the evaluated pixel-splat pairs to a.eval_count (a separate launch for the
+// FLOP count, so that the frame's own blend is timed without the counting).
+template <bool COUNT>
 __global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArgs a,
                                                              const BatchViewDev* __restrict__ views,
                                                              const uint32_t* __restrict__ tile_view) {
@@ -308,12 +354,16 @@ __global__ __launch_bounds__(kTilePixels) void k_batch_blend(const BatchBlendArg
     if (a.counters[kCtrOverflow]) return;  // the frame is rendered again with larger buffers
     const uint32_t tile = a.tile_order ? a.tile_order[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
     const TileView tv = batch_tile_view(views, tile_view, tile);
-    blend_tile<false>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, nullptr, a.final_T, a.final_last);
+    blend_tile<COUNT>(lds, tv, a.ranges[tile], a.vals, a.splats, a.image, a.rgba8, a.eval_count, a.final_T,
+                      a.final_last);
 }
 
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s) {
     if (a.ntiles == 0) return;
-    hipLaunchKernelGGL(k_batch_blend, dim3(a.ntiles), dim3(kTilePixels), 0, s, a, a.views, a.tile_view);
+    if (a.eval_count)
+        hipLaunchKernelGGL(k_batch_blend<true>, dim3(a.ntiles), dim3(kTilePixels), 0, s, a, a.views, a.tile_view);
+    else
+        hipLaunchKernelGGL(k_batch_blend<false>, dim3(a.ntiles), dim3(kTilePixels), 0, s, a, a.views, a.tile_view);
 }
 
 __global__ __launch_bounds__(kTilePixels) void k_batch_blend_bwd(const BatchBlendArgs a,

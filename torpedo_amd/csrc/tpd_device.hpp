@@ -666,6 +666,26 @@ __device__ __forceinline__ void emit_own_rect(uint32_t tiles, uint64_t rect, uin
     }
 }
 
+// The same pairs into a staging chunk: pair k (block-local position first + k) is placed at slot
+// first + k - c0 when that lies in [0, n).
+__device__ __forceinline__ void stage_own_rect(uint32_t tiles, uint64_t rect, uint32_t first, uint32_t c0,
+                                               uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t val) {
+    if (first + tiles <= c0 || first >= c0 + n) return;
+    const uint32_t w = emit_rect_w(rect), gx = emit_rect_gx(rect);
+    uint32_t row = emit_rect_tile0(rect), x = 0;
+    for (uint32_t k = 0; k < tiles; ++k) {
+        const uint32_t slot = first + k - c0;  // wraps past 2^32 below c0: then >= n
+        if (slot < n) {
+            keys[slot] = row + x;
+            vals[slot] = val;
+        }
+        if (++x == w) {
+            x = 0;
+            row += gx;
+        }
+    }
+}
+
 // The large items of a block: s_big_incl[o] is the inclusive count of large tiles over the block's
 // items 0..o (n of them), s_start/s_rect/s_val the block-local first pair, record and value of a
 // large item o.

@@ -198,6 +198,7 @@ struct BatchBlendArgs {
     float* partials;              // backward: P x 9 per pair
     float* final_T;               // per pixel final transmittance: written by a forward kept for the
     int32_t* final_last;          // backward, read by it (with the last blended splat); or null
+    uint64_t* eval_count;         // forward: evaluated pixel-splat pairs are added here, or null
 };
 
 struct BatchGradsDev {            // one scene's parameter-gradient buffers

@@ -640,6 +640,13 @@ class GaussianEngine:
                 "pair_passes_ms": float(ms[3]), "items": int(n[0]), "item_passes": int(n[1]), "pairs": int(n[2]),
                 "pair_passes": int(n[3])}
 
+    def batchBlendEvaluations(self) -> int:
+        """Pixel-splat pairs whose power the last batch frame's blend evaluated
+        (tpd_batch_get_blend_evaluations: the frame is blended once more by a counting kernel)."""
+        ev = ctypes.c_uint64()
+        check(lib().tpd_batch_get_blend_evaluations(self._ctx, ctypes.byref(ev)))
+        return int(ev.value)
+
     def autotuneBatch(self) -> dict:
         ms = np.zeros(64, np.float32)
         ns, best = ctypes.c_uint32(), ctypes.c_uint32()
