@@ -21,8 +21,9 @@
  * exp(): the blend needs exp() in FP32.  The reference leaves it to the Vulkan driver, which
  * cannot run here; this oracle and the HIP kernels both use tpo_expf below (Cody-Waite
  * argument reduction, W. J. Cody and W. Waite, "Software Manual for the Elementary Functions",
- * 1980, followed by the degree-7 Taylor polynomial of e^r evaluated with fused multiply-adds),
- * so that threshold decisions (alpha < 1/255, T < 1e-4) cannot flip between CPU and GPU.
+ * 1980, followed by a degree-5 polynomial of e^r fitted for relative error and evaluated with
+ * fused multiply-adds), so that threshold decisions (alpha < 1/255, T < 1e-4) cannot flip between
+ * CPU and GPU.
  *
  * Parity status of each function is listed in the repository's design notes; the camera
  * functions are additionally pinned bit-for-bit against the reference's own math headers
@@ -51,21 +52,21 @@ static inline int32_t sat_i32(double v) {
 float tpo_expf(float x) {
     if (!(x >= -86.0f)) return 0.0f;
     if (x > 88.0f) return INFINITY;
-    const float n = rintf(x * 1.44269504088896341f);
+    /* n = round(x / ln 2) from the low bits of x log2(e) + 1.5 * 2^23, r = x - n ln 2 (Cody-Waite,
+     * exact), e^r by a degree-5 relative-error fit on [-ln2/2, ln2/2], 2^n added to the exponent */
+    union { float f; uint32_t u; } t, p;
+    t.f = fmaf(x, 1.44269504088896341f, 12582912.0f);
+    const float n = t.f - 12582912.0f;
     float r = fmaf(n, -0.693145751953125f, x);
     r = fmaf(n, -1.42860682030941723212e-6f, r);
-    float p = 1.0f / 5040.0f;
-    p = fmaf(p, r, 1.0f / 720.0f);
-    p = fmaf(p, r, 1.0f / 120.0f);
-    p = fmaf(p, r, 1.0f / 24.0f);
-    p = fmaf(p, r, 1.0f / 6.0f);
-    p = fmaf(p, r, 0.5f);
-    p = fmaf(p, r, 1.0f);
-    p = fmaf(p, r, 1.0f);
-    const int32_t e = (int32_t)n + 127;
-    union { uint32_t u; float f; } sc;
-    sc.u = (uint32_t)e << 23;
-    return p * sc.f;
+    p.f = 0.008286180905997753f;
+    p.f = fmaf(p.f, r, 0.04189886897802353f);
+    p.f = fmaf(p.f, r, 0.16667655110359192f);
+    p.f = fmaf(p.f, r, 0.49999144673347473f);
+    p.f = fmaf(p.f, r, 0.9999997615814209f);
+    p.f = fmaf(p.f, r, 1.0f);
+    p.u += t.u << 23;
+    return p.f;
 }
 
 void tpo_expf_vec(const float* x, float* y, uint64_t n) {

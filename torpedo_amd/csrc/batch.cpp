@@ -419,7 +419,8 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     bool items_alt = false;
     if (b->nitems)
         items_alt = launch_sort_pairs32(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
-                                        b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr);
+                                        b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr,
+                                        false);  // only the depth order of the items is used
     else if (events)
         hipEventRecord(b->ev[9], s);
     b->sorted_items = items_alt ? b->d_ivals_alt : b->d_ivals;

@@ -375,13 +375,14 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // Keys out, each to its digit's global start + its place in the partition; the global position
     // of each output slot is kept for the values (the keys' LDS is reused for them).
     const uint32_t nvalid = min(kTileKeys, n - base);
+    const bool keys_out = a.keys_out != nullptr;  // null: a last pass whose sorted keys nobody reads
     uint32_t gpos[ITEMS];
 #pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
         const KT key = L.keys[i];
         gpos[t] = L.adj[digit(key)] + i;
-        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
+        if (keys_out && (full || i < nvalid)) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
@@ -483,7 +484,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist) {
+                         hipEvent_t after_hist, bool keys_out) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -500,7 +501,7 @@ bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint
     uint32_t* vout = vals_alt;
     for (uint32_t p = 0; p < passes; ++p) {
         OnesweepArgs o;
-        o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
+        o.keys_in = kin; o.vals_in = vin; o.keys_out = (keys_out || p + 1 < passes) ? kout : nullptr; o.vals_out = vout;
         o.count_ptr = count_ptr; o.n_imm = n_max;
         o.shift = p * sh.bits;
         o.lo_bits = 32u;

@@ -234,7 +234,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
         };
         // power <= 0 equals !(power > 0) except for NaN, whose exp is 0 and whose alpha therefore
         // fails the 1/255 test: either way the splat is skipped.  Active lanes have power in
-        // [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf's plain range.
+        // [skip, 0] with skip >= log(1/255) - 0.01, inside soft_expf_inrange's range.
         const auto blend_one = [&](uint32_t off, float power, uint64_t act) {
             float alpha = slot4(s.b, off).z * soft_expf_inrange(power);
             alpha = alpha < 0.99f ? alpha : 0.99f;

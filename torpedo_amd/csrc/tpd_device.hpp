@@ -43,67 +43,26 @@ struct EntityMats {
 __device__ __forceinline__ float as_float(uint32_t u) { return __builtin_bit_cast(float, u); }
 __device__ __forceinline__ uint32_t as_uint(float f) { return __builtin_bit_cast(uint32_t, f); }
 
-// exp(x) for x <= 0: Cody-Waite reduction x = n ln2 + r (two-constant split of ln 2), then the
-// degree-7 Taylor polynomial of e^r in Horner form with fused multiply-adds, scaled by 2^n built
-// from the exponent bits.  Below -86 (and for NaN) it returns 0.  Identical bits on CPU and GPU
-// because it uses only correctly rounded operations.
-__device__ __forceinline__ float soft_expf(float x) {
-    // The polynomial runs on an argument clamped to [-87, 89] (which changes nothing inside
-    // [-86, 88]) and the out-of-range results are selected afterwards, so there is no branch.
-    const float xc = __builtin_fminf(__builtin_fmaxf(x, -87.0f), 89.0f);
-    const float n = __builtin_rintf(xc * 1.44269504088896341f);
-    float r = __builtin_fmaf(n, -0.693145751953125f, xc);
-    r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
-    float p = 1.0f / 5040.0f;
-    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
-    p = __builtin_fmaf(p, r, 0.5f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    const int32_t e = (int32_t)n + 127;
-    float y = p * as_float((uint32_t)e << 23);
-    y = (x >= -86.0f) ? y : 0.0f;  // also NaN -> 0
-    return (x > 88.0f) ? __builtin_inff() : y;
-}
-
-// The polynomial and scaling of soft_expf alone: the same bits as soft_expf for x in [-86, 88];
-// garbage (never a fault) outside.
+// exp(x) for x in [-86, 88]: x = n ln2 + r with n = round(x / ln2) taken from the low bits of
+// x log2(e) + 1.5 * 2^23 (one fused multiply-add rounds the product to an integer), r by a
+// Cody-Waite two-constant split of ln 2 (exact), e^r by a degree-5 polynomial fitted for relative
+// error on [-ln2/2, ln2/2] (Horner form with fused multiply-adds), and 2^n added into the result's
+// exponent field (the result is a normal number over that range).  Relative error below
+// 1.5 x 2^-23 against exp for every float in [-86, 0].  Only correctly rounded operations and an
+// integer add, so the oracle's tpo_expf gives the same bits.  Outside [-86, 88] the result is
+// garbage (never a fault): callers keep their arguments inside.
 __device__ __forceinline__ float soft_expf_inrange(float x) {
-    const float n = __builtin_rintf(x * 1.44269504088896341f);
+    const float t = __builtin_fmaf(x, 1.44269504088896341f, 12582912.0f);
+    const float n = t - 12582912.0f;
     float r = __builtin_fmaf(n, -0.693145751953125f, x);
     r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
-    float p = 1.0f / 5040.0f;
-    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
-    p = __builtin_fmaf(p, r, 0.5f);
+    float p = 0.008286180905997753f;
+    p = __builtin_fmaf(p, r, 0.04189886897802353f);
+    p = __builtin_fmaf(p, r, 0.16667655110359192f);
+    p = __builtin_fmaf(p, r, 0.49999144673347473f);
+    p = __builtin_fmaf(p, r, 0.9999997615814209f);
     p = __builtin_fmaf(p, r, 1.0f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    const int32_t e = (int32_t)n + 127;
-    return p * as_float((uint32_t)e << 23);
-}
-
-// soft_expf for the blend, where every lane that uses the result has an argument in
-// [log(1/(255 o)) - 0.01, 0] (o >= 1/255) or NaN: the same bits as soft_expf there, without the
-// clamp and the overflow select that such arguments never reach (other lanes' results are unused).
-__device__ __forceinline__ float soft_expf_blend(float x) {
-    const float n = __builtin_rintf(x * 1.44269504088896341f);
-    float r = __builtin_fmaf(n, -0.693145751953125f, x);
-    r = __builtin_fmaf(n, -1.42860682030941723212e-6f, r);
-    float p = 1.0f / 5040.0f;
-    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
-    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
-    p = __builtin_fmaf(p, r, 0.5f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    p = __builtin_fmaf(p, r, 1.0f);
-    const int32_t e = (int32_t)n + 127;
-    const float y = p * as_float((uint32_t)e << 23);
-    return (x >= -86.0f) ? y : 0.0f;  // NaN -> 0
+    return as_float(as_uint(p) + (as_uint(t) << 23));
 }
 
 // Float -> UNORM8 for the RGBA8 target (blend.slang:104): clamp to [0, 1], scale by 255, round to

@@ -277,9 +277,10 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
                        hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
                        uint32_t lo_base = 0);
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
-// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.
+// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.  With
+// keys_out = false only the values are sorted on the last pass (the sorted keys are not written).
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist = nullptr);
+                         hipEvent_t after_hist = nullptr, bool keys_out = true);
 
 }  // namespace tpd
