@@ -156,7 +156,8 @@ int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per
 int tpd_measure_bandwidth(tpd_ctx* ctx, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks, float* ms);
 
 /* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----
- * Sorts n (key, value) pairs by key (stable); key bits at and above key_bits must be zero.  d_keys_alt /
+ * Sorts n (key, value) pairs by the key's low key_bits bits (stable); bits at and above key_bits do
+ * not take part in the order and are carried with the key.  d_keys_alt /
  * d_vals_alt are scratch of the same size; *in_alt is set to 1 when the result is there. */
 int tpd_sort_pairs(tpd_ctx* ctx, uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_keys_alt, uint32_t* d_vals_alt,
                    uint32_t n, uint32_t key_bits, void* stream, int* in_alt);
@@ -258,8 +259,11 @@ int tpd_batch_get_sort_times(tpd_ctx* ctx, float* ms, uint32_t* counts);
 /* tpd_get_blend_evaluations for the last batch frame: the pixel-splat pairs whose power its blend
  * evaluated (counted by blending that frame once more with a counting kernel; same image). */
 int tpd_batch_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
-/* tpd_autotune for the batch's sort (times whole batch frames per radix-pass block shape). */
+/* tpd_autotune for the batch's two sorts (times whole batch frames per radix-pass block shape and
+ * keeps the fastest shape for each sort); *best = the pair sort's shape. */
 int tpd_batch_autotune(tpd_ctx* ctx, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best);
+/* The radix block shapes of the batch's item sort (by depth) and pair sort (by tile). */
+int tpd_batch_get_sort_shapes(tpd_ctx* ctx, uint32_t* item_shape, uint32_t* pair_shape);
 
 /* ----- point-cloud files -----
  * GaussianPoint::fromModel (volumetric/src/GaussianGeometry.cpp:59-127): reads a 3DGS PLY file (ascii or

@@ -61,6 +61,21 @@ def test_sort_matches_stable_argsort(eng, n, bits, kind):
     assert np.array_equal(v, vals[o])
 
 
+@pytest.mark.parametrize("n,bits", [(70_001, 27), (300_000, 31), (4097, 17)])
+def test_sort_carries_bits_above_key_bits(eng, n, bits):
+    """Bits at and above key_bits ride along with the key without ordering it (the batched frame's
+    item sort carries each item's emission record there): the result equals a stable argsort of
+    the low key_bits bits, with the high bits intact."""
+    rng = np.random.default_rng(n)
+    low = rng.integers(0, 1 << bits, n, dtype=np.uint64) >> np.uint64(rng.integers(0, 6))  # duplicates too
+    keys = low | (rng.integers(1, 1 << (64 - bits), n, dtype=np.uint64) << np.uint64(bits))
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = gpu_sort(eng, keys, vals, bits)
+    o = np.argsort(low, kind="stable")
+    assert np.array_equal(k, keys[o])
+    assert np.array_equal(v, vals[o])
+
+
 def test_sort_empty_and_invalid(eng):
     from torpedo_amd._capi import TpdError
     k, v = gpu_sort(eng, np.zeros(0, np.uint64), np.zeros(0, np.uint32), 32)

@@ -99,6 +99,7 @@ SIGNATURES = {
     "tpd_batch_get_times": (_i32, [_vp, _vp, ctypes.POINTER(_u32)]),
     "tpd_batch_get_sort_times": (_i32, [_vp, _vp, _vp]),
     "tpd_batch_get_blend_evaluations": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "tpd_batch_get_sort_shapes": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_batch_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_load_ply": (_i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(_u32)]),
     "tpd_free_host": (None, [_vp]),

@@ -36,6 +36,7 @@ struct BatchState {
     std::vector<uint32_t> h_scene_views, h_block_scene, h_tile_view;
     uint32_t nblocks = 0, nitems = 0, ntiles = 0, nmats = 0, max_ent = 1;
     uint64_t npix = 0;
+    uint32_t item_shape = 0;  // radix block shape of the item sort (ws.shape: that of the pair sort)
 
     // device tables
     BatchSceneDev* d_scenes = nullptr; size_t cap_scenes = 0;
@@ -53,9 +54,9 @@ struct BatchState {
     uint32_t* d_tiles = nullptr; size_t cap_tiles = 0;
     uint32_t* d_offsets = nullptr; size_t cap_offsets = 0;
     uint64_t* d_rects = nullptr; size_t cap_rects = 0;  // per item emission record
-    uint32_t* d_ikeys = nullptr; size_t cap_ikeys = 0;  // item sort: depth keys and items, ping-pong
-    uint32_t* d_ivals = nullptr; size_t cap_ivals = 0;
-    uint32_t* d_ikeys_alt = nullptr; size_t cap_ikeys_alt = 0;
+    uint64_t* d_ikeys = nullptr; size_t cap_ikeys = 0;  // item sort: depth keys (+ emission records) and
+    uint32_t* d_ivals = nullptr; size_t cap_ivals = 0;  // items, ping-pong
+    uint64_t* d_ikeys_alt = nullptr; size_t cap_ikeys_alt = 0;
     uint32_t* d_ivals_alt = nullptr; size_t cap_ivals_alt = 0;
     uint2* d_ranges = nullptr; size_t cap_ranges = 0;
     uint32_t* d_tile_order = nullptr; size_t cap_tile_order = 0;
@@ -147,6 +148,7 @@ static BatchState* state(tpd_ctx* c) {
     if (!c->batch) {
         c->batch = new BatchState();
         c->batch->ws.shape = c->ws.shape;  // the context's radix block shape until tuned separately
+        c->batch->item_shape = c->ws.shape;
     }
     return c->batch;
 }
@@ -167,6 +169,7 @@ void batch_set_sort_shape(tpd_ctx* c, uint32_t shape) {
     if (!c->batch) return;
     (void)finish_pending(c, c->batch);
     c->batch->ws.shape = shape;
+    c->batch->item_shape = shape;
     destroy_graphs(c->batch);
 }
 
@@ -415,17 +418,22 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     launch_batch_items(a, s);
     launch_batch_item_offsets(a, s);  // frames kept for a backward only (a.offsets set)
     if (events) hipEventRecord(b->ev[1], s);
-    b->item_passes = b->nitems ? sort_shape_passes(b->ws.shape, b->lo_bits()) : 0;
+    b->item_passes = b->nitems ? sort_shape_passes(b->item_shape, b->lo_bits()) : 0;
     bool items_alt = false;
-    if (b->nitems)
-        items_alt = launch_sort_pairs32(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
-                                        b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr,
-                                        false);  // only the depth order of the items is used
-    else if (events)
+    if (b->nitems) {
+        // 64-bit keys ordered by their low lo_bits bits (the depth key); the high word (the item's
+        // emission record) rides along.  The item sort has a block shape of its own.
+        const uint32_t pair_shape = b->ws.shape;
+        b->ws.shape = b->item_shape;
+        items_alt = launch_sort_pairs(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
+                                      b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr);
+        b->ws.shape = pair_shape;
+    } else if (events)
         hipEventRecord(b->ev[9], s);
     b->sorted_items = items_alt ? b->d_ivals_alt : b->d_ivals;
     if (events) hipEventRecord(b->ev[2], s);
     a.sorted_items = b->sorted_items;
+    a.sorted_keys64 = items_alt ? b->d_ikeys_alt : b->d_ikeys;
     launch_batch_emit(a, s);
     if (events) hipEventRecord(b->ev[3], s);
     const uint32_t key_bits = tile_key_bits(b);
@@ -876,6 +884,14 @@ int tpd_batch_read_pairs(tpd_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t ca
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_read_pairs");
 }
 
+int tpd_batch_get_sort_shapes(tpd_ctx* c, uint32_t* item_shape, uint32_t* pair_shape) {
+    if (!c || !item_shape || !pair_shape) return fail(TPD_ERR_INVALID, "tpd_batch_get_sort_shapes: bad argument");
+    const BatchState* b = state(c);
+    *item_shape = b->item_shape;
+    *pair_shape = b->ws.shape;
+    return TPD_OK;
+}
+
 // Blends the last frame again with the counting kernel (same tiles, splats and order; it writes the
 // same image, and not the backward's per-pixel state) and returns the pixel-splat pairs whose power
 // the blend evaluated: the FLOP count of the frame's blend.
@@ -940,7 +956,8 @@ int tpd_batch_get_sort_times(tpd_ctx* c, float* ms, uint32_t* counts) {
 }
 
 // Times every radix-pass block shape on the batch's own items and pairs (both sorts of full
-// eager frames) and keeps the fastest for the batch; outputs as tpd_autotune.
+// eager frames) and keeps the fastest for each sort; sort_ms[shape] = both sorts' time with that
+// shape, *best = the pair sort's shape (tpd_batch_get_sort_shapes gives both).
 int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* n_shapes, uint32_t* best) {
     if (!c) return fail(TPD_ERR_INVALID, "null context");
     hipSetDevice(c->device);
@@ -953,10 +970,11 @@ int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* 
     const bool keep_graph = b->use_graph, keep_profile = c->profile;
     b->use_graph = false;
     c->profile = true;
-    std::vector<float> med(ns, 0.0f);
+    std::vector<float> med(ns, 0.0f), med_items(ns, 0.0f), med_pairs(ns, 0.0f);
     for (uint32_t sh = 0; sh < ns && rc == TPD_OK; ++sh) {
         b->ws.shape = sh;
-        std::vector<float> t;
+        b->item_shape = sh;
+        std::vector<float> t, ti, tp;
         for (int rep = 0; rep < 4 && rc == TPD_OK; ++rep) {
             if ((rc = launch_frame(c, b, 0, nullptr, 0, s)) != TPD_OK) break;
             if ((rc = validate(c, b)) != TPD_OK) break;
@@ -966,22 +984,33 @@ int tpd_batch_autotune(tpd_ctx* c, float* sort_ms, uint32_t capacity, uint32_t* 
                 rc = fail(TPD_ERR_HIP, "tpd_batch_autotune: timing");
                 break;
             }
-            if (rep > 0) t.push_back(ms_items + ms_pairs);
+            if (rep > 0) {
+                t.push_back(ms_items + ms_pairs);
+                ti.push_back(ms_items);
+                tp.push_back(ms_pairs);
+            }
         }
         if (!t.empty()) {
             std::sort(t.begin(), t.end());
+            std::sort(ti.begin(), ti.end());
+            std::sort(tp.begin(), tp.end());
             med[sh] = t[t.size() / 2];
+            med_items[sh] = ti[ti.size() / 2];
+            med_pairs[sh] = tp[tp.size() / 2];
         }
     }
     b->use_graph = keep_graph;
     c->profile = keep_profile;
     if (rc != TPD_OK) return rc;
-    uint32_t bi = 0;
-    for (uint32_t sh = 1; sh < ns; ++sh)
-        if (med[sh] < med[bi]) bi = sh;
-    b->ws.shape = bi;
+    uint32_t bi = 0, bp = 0;
+    for (uint32_t sh = 1; sh < ns; ++sh) {
+        if (med_items[sh] < med_items[bi]) bi = sh;
+        if (med_pairs[sh] < med_pairs[bp]) bp = sh;
+    }
+    b->item_shape = bi;
+    b->ws.shape = bp;
     destroy_graphs(b);
-    if (best) *best = bi;
+    if (best) *best = bp;
     if (sort_ms)
         for (uint32_t sh = 0; sh < ns && sh < capacity; ++sh) sort_ms[sh] = med[sh];
     return TPD_OK;

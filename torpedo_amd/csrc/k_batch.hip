@@ -85,16 +85,16 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, co
         const uint32_t item = vw.item_base + g;
         Projected p;
         const bool vis = project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
-        uint64_t rect = 0;
-        uint32_t key = culled_key;
+        uint64_t key = culled_key;
         if (vis) {
             store_splat(a.splats + item, true, p);
-            rect = pack_emit_rect(vw.tile_base + (uint32_t)p.y0 * vw.gx + (uint32_t)p.x0, vw.gx,
-                                  (uint32_t)(p.x1 - p.x0), (uint32_t)(p.y1 - p.y0));
-            key = (as_uint(p.vz) - a.lo_base) & culled_key;
+            const uint32_t tile0 = vw.tile_base + (uint32_t)p.y0 * vw.gx + (uint32_t)p.x0;
+            const uint32_t w = (uint32_t)(p.x1 - p.x0), h = (uint32_t)(p.y1 - p.y0);
+            const uint32_t r32 = inline_rect(tile0, vw.gx, w, h);
+            if (r32 == kRectLookup) a.rects[item] = pack_emit_rect(tile0, vw.gx, w, h);
+            key = ((uint64_t)r32 << 32) | ((as_uint(p.vz) - a.lo_base) & culled_key);
             depth_out |= depth_outside_window(p.vz);
         }
-        a.rects[item] = rect;
         a.item_keys[item] = key;
         a.item_vals[item] = item;
         if (a.tiles) a.tiles[item] = p.tiles;
@@ -161,16 +161,19 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
         for (uint32_t j = 0; j < kEmitPerThread; j += 4) {  // i0 is a multiple of 4
             const uint4 v = *reinterpret_cast<const uint4*>(a.sorted_items + i0 + j);
             item[j] = v.x; item[j + 1] = v.y; item[j + 2] = v.z; item[j + 3] = v.w;
+            const ulonglong2* k2 = reinterpret_cast<const ulonglong2*>(a.sorted_keys64 + i0 + j);
+            const ulonglong2 k01 = k2[0], k23 = k2[1];
+            rect[j] = k01.x; rect[j + 1] = k01.y; rect[j + 2] = k23.x; rect[j + 3] = k23.y;
         }
-#pragma unroll
-        for (uint32_t j = 0; j < kEmitPerThread; ++j) rect[j] = a.rects[item[j]];
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < kEmitPerThread; ++j) {
             item[j] = i0 + j < a.nitems ? a.sorted_items[i0 + j] : 0u;
-            rect[j] = i0 + j < a.nitems ? a.rects[item[j]] : 0ull;
+            rect[j] = i0 + j < a.nitems ? a.sorted_keys64[i0 + j] : 0ull;
         }
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kEmitPerThread; ++j) rect[j] = rect_of((uint32_t)(rect[j] >> 32), a.rects, item[j]);
     uint32_t t[kEmitPerThread], mine = 0, mine_big = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kEmitPerThread; ++j) {

@@ -63,10 +63,11 @@ __device__ __forceinline__ uint32_t key_of_lane_below(uint32_t k) {
 
 // KT = uint64_t: (tile | depth) keys, histogrammed on their compacted form; KT = uint32_t: keys
 // sorted as they are (lo_bits and lo_base unused).
+// Only the key's low key_bits bits (key_mask) order it; bits above ride along.
 template <uint32_t BITS, typename KT>
 __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys, const uint32_t* count_ptr,
                                                    uint32_t n_imm, uint32_t passes, uint32_t lo_bits, uint32_t lo_base,
-                                                   uint32_t tile_keys,
+                                                   KT key_mask, uint32_t tile_keys,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ desc0,
                                                    uint32_t* __restrict__ tickets) {
     constexpr uint32_t R = 1u << BITS;
@@ -93,6 +94,8 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys, 
 #pragma unroll
             for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] = compact_key(k[u], lo_bits, lo_base);
         }
+#pragma unroll
+        for (uint32_t u = 0; u < kHistUnroll; ++u) k[u] &= key_mask;
 #pragma unroll
         for (uint32_t u = 0; u < kHistUnroll; ++u) {
             const uint32_t i = c0 + u * 256 + tid;
@@ -173,6 +176,7 @@ struct OnesweepArgs {
     uint32_t lo_bits;          // bits of the key's low word (32; 31 for positive depths; 27 in the depth window)
     uint32_t lo_base;          // subtracted from the low word before compaction
     uint32_t drop;             // kNarrow / kNarrowed: low bits of the compacted key left out of the 32-bit key
+    uint32_t digit_mask;       // this pass's digit bits below key_bits (bits at and above it do not order)
     const uint32_t* offsets;   // 2^BITS global exclusive offsets of this pass's digits
     uint32_t* desc;            // this pass's look-back region (parts x 2^BITS), zero on entry
     uint32_t* desc_next;       // the other region, cleared here for the next pass
@@ -291,8 +295,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // each group advances the wave's running digit counter.  Digits are recomputed from the keys
     // wherever they are needed (a few instructions) rather than kept in registers or LDS.
     const auto digit = [&](KT key) -> uint32_t {
-        if constexpr (MODE == kWide) return digit_of<BITS>(compact_key(key, a.lo_bits, a.lo_base), a.shift);
-        else return (key >> (a.shift - a.drop)) & ((1u << BITS) - 1u);
+        if constexpr (MODE == kWide) return digit_of<BITS>(compact_key(key, a.lo_bits, a.lo_base), a.shift) & a.digit_mask;
+        else return (uint32_t)(key >> (a.shift - a.drop)) & a.digit_mask;
     };
     const uint64_t lt = lanemask_lt();
     uint32_t rank[ITEMS];
@@ -375,14 +379,13 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // Keys out, each to its digit's global start + its place in the partition; the global position
     // of each output slot is kept for the values (the keys' LDS is reused for them).
     const uint32_t nvalid = min(kTileKeys, n - base);
-    const bool keys_out = a.keys_out != nullptr;  // null: a last pass whose sorted keys nobody reads
     uint32_t gpos[ITEMS];
 #pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
         const KT key = L.keys[i];
         gpos[t] = L.adj[digit(key)] + i;
-        if (keys_out && (full || i < nvalid)) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
+        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
@@ -422,10 +425,18 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits) {
 
 template <uint32_t BITS, typename KT>
 static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const KT* keys, const uint32_t* count_ptr,
-                        uint32_t n_max, uint32_t lo_bits, uint32_t lo_base, uint32_t tile, SortWorkspace& ws) {
+                        uint32_t n_max, uint32_t lo_bits, uint32_t lo_base, uint32_t key_bits, uint32_t tile,
+                        SortWorkspace& ws) {
+    const KT key_mask = key_bits >= 8 * sizeof(KT) ? ~KT(0) : (KT(1) << key_bits) - KT(1);
     hipLaunchKernelGGL((k_sort_hist<BITS, KT>), dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits,
-                       lo_base, tile, ws.partials, ws.desc, ws.tickets);
+                       lo_base, key_mask, tile, ws.partials, ws.desc, ws.tickets);
     hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+}
+
+// The digit bits of the pass at `shift` that lie below key_bits.
+static uint32_t pass_digit_mask(uint32_t bits, uint32_t key_bits, uint32_t shift) {
+    const uint32_t w = key_bits - shift < bits ? key_bits - shift : bits;
+    return (1u << w) - 1u;
 }
 
 static uint32_t hist_blocks(uint32_t n_max) {
@@ -444,8 +455,8 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
 
     const uint32_t hb = hist_blocks(n_max);
-    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
-    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, tile, ws);
+    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
+    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
     // With narrow_drop, the first pass whose digit starts at or above key_bits - 32 writes 32-bit
     // keys (the bits from its digit up) and the later passes move 8 instead of 12 bytes per pair.
@@ -470,6 +481,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
         o.lo_bits = lo_bits;
         o.lo_base = lo_base;
         o.drop = narrow_pass < passes ? narrow_pass * sh.bits : 0u;
+        o.digit_mask = pass_digit_mask(sh.bits, key_bits, o.shift);
         const uint32_t mode = p < narrow_pass ? kWide : (p == narrow_pass ? kNarrow : kNarrowed);
         o.offsets = ws.offsets + p * R;
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
@@ -484,7 +496,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist, bool keys_out) {
+                         hipEvent_t after_hist) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -492,8 +504,8 @@ bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint
     const uint32_t parts = (n_max + tile - 1) / tile;
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
     const uint32_t hb = hist_blocks(n_max);
-    if (sh.bits == 8) launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, tile, ws);
-    else launch_hist<9>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, tile, ws);
+    if (sh.bits == 8) launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
+    else launch_hist<9>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
     if (after_hist) hipEventRecord(after_hist, s);
     const uint32_t* kin = keys;
     const uint32_t* vin = vals;
@@ -501,12 +513,13 @@ bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint
     uint32_t* vout = vals_alt;
     for (uint32_t p = 0; p < passes; ++p) {
         OnesweepArgs o;
-        o.keys_in = kin; o.vals_in = vin; o.keys_out = (keys_out || p + 1 < passes) ? kout : nullptr; o.vals_out = vout;
+        o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
         o.count_ptr = count_ptr; o.n_imm = n_max;
         o.shift = p * sh.bits;
         o.lo_bits = 32u;
         o.lo_base = 0u;
         o.drop = 0u;
+        o.digit_mask = pass_digit_mask(sh.bits, key_bits, o.shift);
         o.offsets = ws.offsets + p * R;
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
         o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * R;

@@ -609,6 +609,23 @@ __host__ __device__ inline uint32_t emit_rect_gx(uint64_t r) { return (uint32_t)
 __host__ __device__ inline uint32_t emit_rect_w(uint64_t r) { return (uint32_t)(r >> 42) & 0x7FFu; }
 __host__ __device__ inline uint32_t emit_rect_h(uint64_t r) { return (uint32_t)(r >> 53) & 0x7FFu; }
 
+// An item's emission record as carried through the item sort in the high word of its 64-bit key
+// (the sort orders by the low word alone): 0 for a culled item; for a rectangle of at most 4 x 4
+// tiles in a view at most 127 tiles wide whose first global tile is below 2^21,
+// tile0 | (w - 1) << 21 | (h - 1) << 23 | gx << 25; for any other, kRectLookup (the full record is
+// then in the item's slot of the per-item records).
+constexpr uint32_t kRectLookup = 1u;
+__host__ __device__ inline uint32_t inline_rect(uint32_t tile0, uint32_t gx, uint32_t w, uint32_t h) {
+    return (w <= 4u && h <= 4u && gx <= 127u && tile0 < (1u << 21))
+               ? tile0 | ((w - 1u) << 21) | ((h - 1u) << 23) | (gx << 25)
+               : kRectLookup;
+}
+__device__ __forceinline__ uint64_t rect_of(uint32_t u, const uint64_t* __restrict__ rects, uint32_t item) {
+    if (u == 0u) return 0ull;
+    if ((u >> 25) == 0u) return rects[item];
+    return pack_emit_rect(u & 0x1FFFFFu, u >> 25, ((u >> 21) & 3u) + 1u, ((u >> 23) & 3u) + 1u);
+}
+
 __device__ __forceinline__ void emit_own_rect(uint32_t tiles, uint64_t rect, uint64_t pos, uint64_t cap,
                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t val) {
     const uint32_t w = emit_rect_w(rect), gx = emit_rect_gx(rect);

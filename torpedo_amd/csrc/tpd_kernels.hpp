@@ -171,11 +171,15 @@ struct BatchArgs {
     uint32_t* tiles;              // per item tile counts, or null (kept for the backward)
     uint32_t* offsets;            // per item exclusive scan of the tile counts in item order, or null:
                                   // where the backward keeps the item's per-pair partial sums
-    uint64_t* rects;              // per item emission record (pack_emit_rect); zero if culled
-    uint32_t* item_keys;          // per item depth key: (depth bits - lo_base) in lo_bits bits, all ones
-    uint32_t* item_vals;          //   within lo_bits when culled; and the item itself
+    uint64_t* rects;              // per item emission record (pack_emit_rect), written only where the key
+                                  // carries kRectLookup
+    uint64_t* item_keys;          // per item: low word the depth key ((depth bits - lo_base) in lo_bits
+                                  // bits, all ones within lo_bits when culled), high word the emission
+                                  // record as inline_rect encodes it
+    uint32_t* item_vals;          // per item: the item itself
     uint32_t lo_bits, lo_base;
-    const uint32_t* sorted_items; // emission input: the items in depth-key order
+    const uint64_t* sorted_keys64;  // emission input: the item keys in depth order ...
+    const uint32_t* sorted_items;   // ... and their items
     uint32_t* keys;               // pairs: global tile
     uint32_t* vals;               //        item
     uint2* ranges;                // all views' tiles, cleared by the item kernel
@@ -277,10 +281,9 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
                        hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
                        uint32_t lo_base = 0);
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
-// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.  With
-// keys_out = false only the values are sorted on the last pass (the sorted keys are not written).
+// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist = nullptr, bool keys_out = true);
+                         hipEvent_t after_hist = nullptr);
 
 }  // namespace tpd

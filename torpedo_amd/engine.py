@@ -651,7 +651,10 @@ class GaussianEngine:
         ms = np.zeros(64, np.float32)
         ns, best = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().tpd_batch_autotune(self._ctx, ptr(ms), 64, ctypes.byref(ns), ctypes.byref(best)))
-        return {"sort_ms_per_shape": [round(float(x), 4) for x in ms[: ns.value]], "best_shape": int(best.value)}
+        item, pair = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().tpd_batch_get_sort_shapes(self._ctx, ctypes.byref(item), ctypes.byref(pair)))
+        return {"sort_ms_per_shape": [round(float(x), 4) for x in ms[: ns.value]], "best_shape": int(best.value),
+                "item_shape": int(item.value), "pair_shape": int(pair.value)}
 
 
 def sort_pairs_device(engine: GaussianEngine, keys_ptr: int, vals_ptr: int, keys_alt_ptr: int, vals_alt_ptr: int,
