@@ -37,8 +37,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per element per k_onesweep launch of the batched frame (32-bit keys and values; mean
 # over the frame's item and pair passes), per radix shape, from two rocprofv3 --pmc runs:
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors read off a plain copy kernel).
-ONESWEEP_TRAFFIC_B_PER_ELEMENT = {}
-ONESWEEP_TRAFFIC_SOURCE = {}
+# HBM bytes per k_onesweep launch of the batched frame (mean over its launches), measured with
+# FETCH_SIZE x 2 + WRITE_SIZE for the (item sort, pair sort) block shapes named; other shapes: null.
+ONESWEEP_TRAFFIC = {
+    (6, 8): {"bytes_per_launch": 1_024_400_000,
+             "source": "profiles/batch_frame-multiview32-clusters1M.md (3 item passes x 935.1 MB + 2 pair "
+                       "passes x 1,158.3 MB, FETCH_SIZE x 2 + WRITE_SIZE)"},
+}
 VALU_PEAK_TFLOPS = 157.3
 
 
@@ -332,7 +337,8 @@ def time_multiscene(eng, steps, warmup):
 def hbm_ceiling(eng):
     """Achievable HBM bandwidth on this GPU: copy / hipMemcpyAsync / nontemporal copy / read-only /
     write-only streams of 256 MiB .. 4 GiB, kernels at 2048 .. 32768 blocks of 256 threads (GB/s,
-    copies counting read + written bytes)."""
+    copies counting read + written bytes); and the rate of random 8- and 4-byte reads from arrays
+    of 32 MiB .. 2 GiB (G reads/s)."""
     out = {}
     for nbytes in (256 << 20, 1 << 30, 4 << 30):
         for kind in eng.BANDWIDTH_KINDS:
@@ -340,7 +346,13 @@ def hbm_ceiling(eng):
                 key = f"{kind}/{nbytes >> 20}MiB" + (f"/{blocks}blocks" if blocks else "")
                 out[key] = round(eng.measureBandwidth(kind, nbytes, 20, blocks), 1)
                 log(f"[bench] {key}: {out[key]} GB/s")
-    return {"hbm_ceiling_gbs": out, "peak_gbs": HBM_PEAK_GBS}
+    gathers = {}
+    for nbytes in (32 << 20, 128 << 20, 256 << 20, 512 << 20, 2 << 30):
+        for kind in eng.GATHER_KINDS:
+            key = f"{kind}/{nbytes >> 20}MiB"
+            gathers[key] = round(eng.measureGather(kind, nbytes), 2)
+            log(f"[bench] {key}: {gathers[key]} G reads/s")
+    return {"hbm_ceiling_gbs": out, "random_reads_g_per_s": gathers, "peak_gbs": HBM_PEAK_GBS}
 
 
 def cpu_baseline(g, views, budget_s=12.0):
@@ -450,23 +462,34 @@ def main():
     del g2
     log(f"[bench] sort-load: P={P2}, stages {st2}")
 
-    # Roofline of the radix passes (k_onesweep, 32-bit keys) of the batched frame: the items are
-    # sorted by depth, then the pairs by tile; 16 algorithmic bytes per element per pass (4-B key
-    # and 4-B value, read and written), achieved = all passes' bytes / all passes' time (the
-    # time-weighted mean over the frame's launches).
+    # Roofline of the radix passes (k_onesweep) of the batched frame: the items are sorted by depth
+    # (64-bit keys: the depth key and the carried emission record; 32-bit values: 24 algorithmic
+    # bytes per element per pass, read and written), then the pairs by tile (32-bit keys and
+    # values: 16 bytes); achieved = all passes' bytes / all passes' time (the time-weighted mean over
+    # the frame's launches).
     es, so = mv["eager_stage_ms"], mv["sorts"]
-    shape_b = (mv.get("autotune") or {}).get("best_shape", args.sort_shape)
-    pass_bytes = 16.0 * (so["items"] * so["item_passes"] + so["pairs"] * so["pair_passes"])
+    tuned = mv.get("autotune") or {}
+    item_shape = tuned.get("item_shape", args.sort_shape)
+    pair_shape = tuned.get("pair_shape", args.sort_shape)
+    item_bytes = 24.0 * so["items"] * so["item_passes"]
+    pair_bytes = 16.0 * so["pairs"] * so["pair_passes"]
+    pass_bytes = item_bytes + pair_bytes
     t_passes = (so["item_passes_ms"] + so["pair_passes_ms"]) / 1000.0
     launches = so["item_passes"] + so["pair_passes"]
     ach = pass_bytes / t_passes / 1e9 if t_passes > 0 else 0.0
-    traffic_b = ONESWEEP_TRAFFIC_B_PER_ELEMENT.get(shape_b)
+    traffic = ONESWEEP_TRAFFIC.get((item_shape, pair_shape))
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": (round(traffic_b * pass_bytes / 16.0 / launches) if traffic_b else None),
-                "traffic_source": ONESWEEP_TRAFFIC_SOURCE.get(shape_b), "kernel": "k_onesweep (32-bit keys)",
-                "sort_shape": shape_desc(shape_b), "items": so["items"], "item_passes": so["item_passes"],
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "kernel": "k_onesweep (item sort: 64-bit keys; pair sort: 32-bit keys)",
+                "sort_shape": {"items": shape_desc(item_shape), "pairs": shape_desc(pair_shape)},
+                "items": so["items"], "item_passes": so["item_passes"],
                 "pairs": so["pairs"], "pair_passes": so["pair_passes"],
+                "item_passes_gbs": round(item_bytes / (so["item_passes_ms"] / 1000.0) / 1e9, 1)
+                if so["item_passes_ms"] > 0 else 0.0,
+                "pair_passes_gbs": round(pair_bytes / (so["pair_passes_ms"] / 1000.0) / 1e9, 1)
+                if so["pair_passes_ms"] > 0 else 0.0,
                 "algorithmic_bytes_per_launch": round(pass_bytes / max(launches, 1)),
                 "avg_launch_us": round(t_passes / max(launches, 1) * 1e6, 2),
                 "workload": "multiview-32-clusters-1M (batched frame, issued eagerly for stage events)"}
@@ -474,7 +497,7 @@ def main():
     # pairs by (tile, depth): both sorts, histograms included (K = global tile bits + 27 depth bits).
     t_sort = (es["item_sort"] + es["pair_sort"]) / 1000.0
     K = int(np.ceil(np.log2(max(sum(((v[0] + 15) // 16) * ((v[1] + 15) // 16) for v in mv["resolutions"]), 2)))) + 27
-    d = SORT_SHAPES[shape_b][0] if shape_b is not None and 0 <= shape_b < len(SORT_SHAPES) else 9
+    d = SORT_SHAPES[pair_shape][0] if pair_shape is not None and 0 <= pair_shape < len(SORT_SHAPES) else 9
     sort_metric = 24.0 * mv["pairs"] * ((K + d - 1) // d) / t_sort / 1e9 if t_sort > 0 else 0.0
     roofline["sort_ms"] = round(t_sort * 1000.0, 4)
     roofline["sort_gkeys_per_s"] = round(mv["pairs"] / t_sort / 1e9, 3) if t_sort > 0 else 0.0
@@ -520,7 +543,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 scenes and camera rigs, no dataset)",
         "config": {"workload": "multiview-32-clusters-1M", "gaussians": 1_000_000, "views": 32,
                    "radius": 8.0, "sh_degree": 3, "tile": 16, "pairs": mv["pairs"],
-                   "sort_shape_batch": shape_desc((mv.get("autotune") or {}).get("best_shape", args.sort_shape)),
+                   "sort_shape_batch": {"items": shape_desc((mv.get("autotune") or {}).get("item_shape", args.sort_shape)),
+                                        "pairs": shape_desc((mv.get("autotune") or {}).get("pair_shape", args.sort_shape))},
                    "sort_shape_plain": shape_desc((mv.get("autotune_plain") or {}).get("best_shape", args.sort_shape)),
                    "parallelism": "single GPU", "graph": "one recorded HIP graph per frame"},
         "multiview_32": mv,

@@ -152,7 +152,8 @@ int tpd_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
 int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per_copy);
 /* Achievable bandwidth of a stream of `bytes`: kind 0 copy kernel, 1 hipMemcpyAsync device to device,
  * 2 nontemporal copy kernel, 3 read-only kernel, 4 write-only kernel; `blocks` is the kernels' grid
- * (0: 8192 blocks of 256 threads).  *ms is the mean time per pass over iters passes. */
+ * (0: 8192 blocks of 256 threads).  Kinds 5 and 6: 2^25 random 8-byte / 4-byte reads from an array
+ * of `bytes` (the rate of a gather by a sorted index).  *ms is the mean time per pass over iters. */
 int tpd_measure_bandwidth(tpd_ctx* ctx, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks, float* ms);
 
 /* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----

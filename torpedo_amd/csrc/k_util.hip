@@ -1,6 +1,6 @@
-// Measurement helpers: plain streaming kernels (copy, nontemporal copy, read-only, write-only) and
-// hipMemcpyAsync, used to calibrate HBM counters and to state the achievable bandwidth beside the
-// kernels' roofline fractions.
+// Measurement helpers: plain streaming kernels (copy, nontemporal copy, read-only, write-only),
+// hipMemcpyAsync and random gathers, used to calibrate HBM counters and to state the achievable
+// bandwidth and random-read rate beside the kernels' roofline fractions.
 #include "tpd_ctx.hpp"
 
 namespace tpd {
@@ -60,24 +60,48 @@ __global__ __launch_bounds__(256) void k_fill(float4* __restrict__ dst, size_t n
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = f;
 }
 
+// Random reads: out[i] = src[h(i) mod n_src] for kGatherReads indices h(i) of a 64-bit mix, four
+// independent reads in flight per thread (the pattern of a gather by a sorted index).
+constexpr uint32_t kGatherReads = 1u << 25;
+__device__ __forceinline__ uint32_t gather_index(uint32_t i, uint32_t n_src) {
+    uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)((z ^ (z >> 31)) % n_src);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ src, uint32_t n_src, T* __restrict__ out) {
+    const uint32_t i0 = (blockIdx.x * 256 + threadIdx.x) * 4;  // the grid covers kGatherReads exactly
+    const T a = src[gather_index(i0, n_src)], b = src[gather_index(i0 + 1, n_src)];
+    const T c = src[gather_index(i0 + 2, n_src)], d = src[gather_index(i0 + 3, n_src)];
+    out[i0] = a;
+    out[i0 + 1] = b;
+    out[i0 + 2] = c;
+    out[i0 + 3] = d;
+}
+
 }  // namespace tpd
 
 using namespace tpd;
 
 // kind: 0 copy kernel, 1 hipMemcpyAsync device to device, 2 nontemporal copy kernel, 3 read-only
-// kernel, 4 write-only kernel.  blocks: grid of the kernels (0: 8192).  *ms: mean time per pass.
+// kernel, 4 write-only kernel, 5 / 6 random 8-byte / 4-byte reads (2^25 of them from an array of
+// `bytes`).  blocks: grid of the streaming kernels (0: 8192).  *ms: mean time per pass.
 extern "C" int tpd_measure_bandwidth(tpd_ctx* c, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks,
                                      float* ms) {
-    if (!c || !ms || bytes < 16 || iters == 0 || kind > 4)
+    if (!c || !ms || bytes < 16 || iters == 0 || kind > 6)
         return fail(TPD_ERR_INVALID, "tpd_measure_bandwidth: bad argument");
+    if (kind >= 5 && bytes / 16 * (kind == 5 ? 2u : 4u) > 0xFFFFFFFFull)
+        return fail(TPD_ERR_INVALID, "tpd_measure_bandwidth: gather array above 2^32 elements");
     hipSetDevice(c->device);
     const size_t n = bytes / 16;
     const uint32_t grid = blocks ? blocks : 256 * 8 * 4;
     float4 *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipMalloc(&a, n * 16);
-    if (e == hipSuccess && kind != 3 && kind != 4) e = hipMalloc(&b, n * 16);
+    if (e == hipSuccess && kind <= 2) e = hipMalloc(&b, n * 16);
     if (e == hipSuccess && kind == 3) e = hipMalloc(&b, 256 * sizeof(float));
+    if (e == hipSuccess && kind >= 5) e = hipMalloc(&b, (size_t)kGatherReads * 8);
     if (e == hipSuccess) e = hipMemsetAsync(a, 1, n * 16, c->stream);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -90,7 +114,15 @@ extern "C" int tpd_measure_bandwidth(tpd_ctx* c, uint32_t kind, uint64_t bytes, 
                                    reinterpret_cast<f4v*>(b), n);
                 break;
             case 3: hipLaunchKernelGGL(k_read, dim3(grid), dim3(256), 0, c->stream, a, reinterpret_cast<float*>(b), n); break;
-            default: hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, c->stream, a, n, 1.0f); break;
+            case 4: hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, c->stream, a, n, 1.0f); break;
+            case 5:
+                hipLaunchKernelGGL(k_gather<uint64_t>, dim3(kGatherReads / 1024), dim3(256), 0, c->stream,
+                                   reinterpret_cast<const uint64_t*>(a), (uint32_t)(n * 2), reinterpret_cast<uint64_t*>(b));
+                break;
+            default:
+                hipLaunchKernelGGL(k_gather<uint32_t>, dim3(kGatherReads / 1024), dim3(256), 0, c->stream,
+                                   reinterpret_cast<const uint32_t*>(a), (uint32_t)(n * 4), reinterpret_cast<uint32_t*>(b));
+                break;
         }
         return hipGetLastError();
     };

@@ -468,6 +468,7 @@ class GaussianEngine:
         return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     BANDWIDTH_KINDS = ("copy", "memcpy", "copy_nontemporal", "read", "write")
+    GATHER_KINDS = {"gather8": 5, "gather4": 6}
 
     def measureBandwidth(self, kind: str, nbytes: int, iters: int = 20, blocks: int = 0) -> float:
         """GB/s of one streaming pattern over nbytes (tpd_measure_bandwidth); copies count the bytes
@@ -477,6 +478,14 @@ class GaussianEngine:
         check(lib().tpd_measure_bandwidth(self._ctx, k, int(nbytes), int(iters), int(blocks), ctypes.byref(ms)))
         moved = 2.0 * nbytes if k <= 2 else float(nbytes)
         return moved / (ms.value * 1e-3) / 1e9
+
+    def measureGather(self, kind: str, nbytes: int, iters: int = 10) -> float:
+        """Random reads per second (G/s): 2^25 random 8-byte ("gather8") or 4-byte ("gather4") reads
+        from an array of nbytes (tpd_measure_bandwidth kinds 5 and 6)."""
+        ms = ctypes.c_float()
+        check(lib().tpd_measure_bandwidth(self._ctx, self.GATHER_KINDS[kind], int(nbytes), int(iters), 0,
+                                          ctypes.byref(ms)))
+        return (1 << 25) / (ms.value * 1e-3) / 1e9
 
     def deviceImagePtr(self) -> int:
         p = ctypes.c_void_p()
