@@ -47,6 +47,8 @@ def lib():
         L.tpo_expf.restype = ctypes.c_float
         L.tpo_expf.argtypes = [ctypes.c_float]
         L.tpo_expf_vec.argtypes = [_f32p, _f32p, ctypes.c_uint64]
+        L.tpo_expf_sweep.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
         L.tpo_look_at.argtypes = [_f32p, _f32p, _f32p, _f32p]
         L.tpo_perspective.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, _f32p]
@@ -184,6 +186,14 @@ def expf(x) -> np.ndarray:
     y = np.empty_like(x)
     lib().tpo_expf_vec(x, y, x.size)
     return y
+
+
+def expf_sweep(lo: float, hi: float, nthreads: int = 8):
+    """Every float of [lo, hi] (lo < hi <= 0): (largest relative error of expf against the
+    double-precision exp, number of neighbouring floats whose results decrease)."""
+    m, d = ctypes.c_double(), ctypes.c_uint64()
+    lib().tpo_expf_sweep(float(lo), float(hi), int(nthreads), ctypes.byref(m), ctypes.byref(d))
+    return m.value, d.value
 
 
 # ------------------------------------------------------------------ stages

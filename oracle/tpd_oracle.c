@@ -98,6 +98,57 @@ static void parallel_for(range_fn fn, void* arg, uint64_t n, int nthreads) {
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
 }
 
+/* Every float x with lo <= x <= hi (lo < hi <= 0, lo > -87): the largest relative error of tpo_expf
+ * against the double-precision exp, and the number of neighbouring floats whose results decrease
+ * (*down), over nthreads threads. */
+typedef struct { uint32_t u0; uint64_t lo, hi; double max_rel; uint64_t down; } expsweep_t;
+static void* expsweep_main(void* p) {
+    expsweep_t* s = (expsweep_t*)p;
+    double m = 0.0;
+    uint64_t down = 0;
+    float prev = 0.0f;
+    for (uint64_t k = s->lo; k < s->hi; ++k) {
+        union { uint32_t u; float f; } x;
+        x.u = s->u0 - (uint32_t)k;  /* from lo upwards: negative floats grow as their bit pattern shrinks */
+        const float y = tpo_expf(x.f);
+        const double ref = exp((double)x.f);
+        const double e = fabs((double)y - ref) / ref;
+        if (e > m) m = e;
+        if (k > s->lo && y < prev) ++down;
+        prev = y;
+    }
+    s->max_rel = m;
+    s->down = down;
+    return NULL;
+}
+
+void tpo_expf_sweep(float lo, float hi, int nthreads, double* max_rel, uint64_t* down) {
+    union { float f; uint32_t u; } a, b;
+    a.f = lo;
+    b.f = hi;
+    if (b.u == 0u) b.u = 0x80000000u;  /* +0 -> -0 */
+    const uint64_t n = (uint64_t)(a.u - b.u) + 1;  /* floats from lo up to hi */
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    expsweep_t ch[256];
+    for (int t = 0; t < nthreads; ++t) {
+        ch[t].u0 = a.u;
+        ch[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
+        ch[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        /* one float of overlap so that the monotonicity check also covers the chunk boundaries */
+        if (t > 0) ch[t].lo -= 1;
+        pthread_create(&th[t], NULL, expsweep_main, &ch[t]);
+    }
+    *max_rel = 0.0;
+    *down = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        pthread_join(th[t], NULL);
+        if (ch[t].max_rel > *max_rel) *max_rel = ch[t].max_rel;
+        *down += ch[t].down;
+    }
+}
+
 /* ---------------------------------------------------------------- camera (host math) */
 
 /* math/common.h:11-17: a*b - c*d with Kahan's FMA error term. */
