@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import subprocess
 import os
 import sys
 import time
@@ -355,6 +356,26 @@ def hbm_ceiling(eng):
     return {"hbm_ceiling_gbs": out, "random_reads_g_per_s": gathers, "peak_gbs": HBM_PEAK_GBS}
 
 
+def sort_vs_rocprim():
+    """tests/cpp/sort_vs_rocprim (built by __graft_entry__.build()): the stand-alone sort under every
+    block shape against rocPRIM's radix_sort_pairs, same keys, values and key bits."""
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "cpp", "sort_vs_rocprim")
+    if not os.path.exists(exe):
+        raise SystemExit("tests/cpp/sort_vs_rocprim is not built (run __graft_entry__.build())")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise SystemExit(f"sort_vs_rocprim failed ({r.returncode}): {r.stderr.strip()}")
+    cases = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    for c in cases:
+        best = min(range(len(c["tpd_ms_per_shape"])), key=lambda i: c["tpd_ms_per_shape"][i])
+        c["tpd_best_shape"] = shape_desc(best)
+        c["tpd_best_ms"] = c["tpd_ms_per_shape"][best]
+        c["rocprim_over_tpd"] = round(c["rocprim_ms"] / c["tpd_best_ms"], 3)
+        log(f"[bench] sort vs rocPRIM: {c['pairs']} pairs, {c['key_bits']} bits: rocPRIM {c['rocprim_ms']} ms, "
+            f"this sort {c['tpd_best_ms']} ms ({c['tpd_best_shape']})")
+    return {"sort_vs_rocprim": cases}
+
+
 def cpu_baseline(g, views, budget_s=12.0):
     """The oracle (multithreaded C restatement) on the first views of the same 32-view workload,
     until about budget_s seconds of CPU work; views per second."""
@@ -387,7 +408,7 @@ def main():
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
     ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene", "pipelined",
-                                       "hbm-ceiling"],
+                                       "hbm-ceiling", "sort-vs-rocprim"],
                     default=None,
                     help="run one leg only (for profiler runs)")
     args = ap.parse_args()
@@ -433,6 +454,10 @@ def main():
     if args.only == "hbm-ceiling":
         print(json.dumps(hbm_ceiling(eng)), flush=True)
         eng.close()
+        return
+    if args.only == "sort-vs-rocprim":
+        eng.close()
+        print(json.dumps(sort_vs_rocprim()), flush=True)
         return
     if args.only == "pipelined":
         g = scenes.cluster_scene(1_000_000, seed=2)

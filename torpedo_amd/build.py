@@ -67,6 +67,22 @@ def build_cpp_test() -> list[str]:
     return out
 
 
+def build_sort_compare() -> str:
+    """tests/cpp/sort_vs_rocprim: the library's stand-alone sort against rocPRIM's radix sort."""
+    src = os.path.join(ROOT, "tests", "cpp", "sort_vs_rocprim.hip")
+    exe = os.path.join(ROOT, "tests", "cpp", "sort_vs_rocprim")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-Wno-unused-result",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+           "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-ltpd_hip",
+           "-Wl,-rpath,$ORIGIN/../../torpedo_amd", "-o", exe + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building tests/cpp/sort_vs_rocprim.hip")
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     cmd = command(LIB + ".tmp")
     if verbose:
