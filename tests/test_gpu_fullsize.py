@@ -166,7 +166,7 @@ def test_clusters_3m_backward(engines, orc, scene_3m):
             assert np.all(np.isfinite(gr)), name
             assert not gr[culled].any(), name
             assert np.abs(gr[~culled]).max() > 0, name
-        # the same view as a one-view batch: same images, the same gradients up to summation order
+        # the same view as a one-view batch: the same image and the same gradients, bit for bit
         beng.clearScenes()
         beng.addScene(g)
         beng.setViews([(0, cam.view, cam.proj_view, cam.focal_ndc, W, H)])
