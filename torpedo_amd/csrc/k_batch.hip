@@ -80,6 +80,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, co
         rec[4 * q + 0] = t.x; rec[4 * q + 1] = t.y; rec[4 * q + 2] = t.z; rec[4 * q + 3] = t.w;
     }
     bool depth_out = false;  // a visible depth outside the sort's depth window
+    const OpacityTerms ot = opacity_terms(rec[3]);  // the same for every view
     for (uint32_t k = 0; k < sc.nviews; ++k) {
         const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, co
         const bool vis = project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
         uint64_t key = culled_key;
         if (vis) {
-            store_splat(a.splats + item, true, p);
+            store_splat(a.splats + item, true, p, ot);
             const uint32_t tile0 = vw.tile_base + (uint32_t)p.y0 * vw.gx + (uint32_t)p.x0;
             const uint32_t w = (uint32_t)(p.x1 - p.x0), h = (uint32_t)(p.y1 - p.y0);
             const uint32_t r32 = inline_rect(tile0, vw.gx, w, h);

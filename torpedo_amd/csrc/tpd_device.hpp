@@ -303,10 +303,12 @@ struct SubBlockReach {
     float ey, vstar, aq0, inva;
 };
 
-__host__ __device__ inline SubBlockReach sub_block_reach(float a, float b, float c, float o) {
+// q0 of the mask, which depends on the opacity alone.
+__host__ __device__ inline float mask_q0(float o) { return 1.02f * (mask_log(255.0f * o) + 0.05f) * (2.0f / 0.99f); }
+
+__host__ __device__ inline SubBlockReach sub_block_reach_q0(float a, float b, float c, float o, float q0) {
     SubBlockReach r{-1.0f, 0.0f, 0.0f, 0.0f};
     if (o < 1.0f / 255.0f) return r;  // alpha <= o for power <= 0
-    const float q0 = 1.02f * (mask_log(255.0f * o) + 0.05f) * (2.0f / 0.99f);
     if (!(a > 0.0f && c > 0.0f && b * b < 0.98f * (a * c) && q0 >= 0.0f)) {
         r.ey = __builtin_nanf("");
         return r;
@@ -319,6 +321,10 @@ __host__ __device__ inline SubBlockReach sub_block_reach(float a, float b, float
     r.aq0 = a * q0;
     r.inva = mask_rcp(a);
     return r;
+}
+
+__host__ __device__ inline SubBlockReach sub_block_reach(float a, float b, float c, float o) {
+    return sub_block_reach_q0(a, b, c, o, mask_q0(o));
 }
 
 // The per-tile part: the band extents of the splat at (gx, gy) with conic (a, b, c) against the
@@ -352,20 +358,32 @@ __host__ __device__ inline uint32_t splat_sub_block_mask(float gx, float gy, flo
     return sub_block_mask(gx, gy, a, b, c, sub_block_reach(a, b, c, o), ox, oy);
 }
 
+// The parts of a splat record that depend on the opacity alone (the same for every view of a
+// Gaussian): the blend's skip threshold log(1/(255 o)) - 0.01 and the sub-block mask's q0.
+struct OpacityTerms {
+    float skip, q0;
+};
+__device__ __forceinline__ OpacityTerms opacity_terms(float o) {
+    return OpacityTerms{__logf(1.0f / (255.0f * o)) - 0.01f, mask_q0(o)};
+}
+
 // Writes one splat record: the projected values (and the tile-independent part of the blend's
-// sub-block mask), or zeros for a culled Gaussian.
-__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p) {
+// sub-block mask), or zeros for a culled Gaussian.  ot = opacity_terms(p.opacity).
+__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p, const OpacityTerms& ot) {
     Splat s;
     if (vis) {
-        const SubBlockReach r = sub_block_reach(p.con_a, p.con_b, p.con_c, p.opacity);
+        const SubBlockReach r = sub_block_reach_q0(p.con_a, p.con_b, p.con_c, p.opacity, ot.q0);
         s.xyzr = make_float4(p.px, p.py, p.vz, p.radius);
         s.conop = make_float4(p.con_a, p.con_b, p.con_c, p.opacity);
         s.rgb = make_float4(p.r, p.g, p.b, r.inva);
-        s.aux = make_float4(__logf(1.0f / (255.0f * p.opacity)) - 0.01f, r.ey, r.vstar, r.aq0);
+        s.aux = make_float4(ot.skip, r.ey, r.vstar, r.aq0);
     } else {
         s.xyzr = s.conop = s.rgb = s.aux = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     *dst = s;
+}
+__device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projected& p) {
+    store_splat(dst, vis, p, vis ? opacity_terms(p.opacity) : OpacityTerms{0.0f, 0.0f});
 }
 
 

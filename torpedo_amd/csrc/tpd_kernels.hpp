@@ -163,7 +163,7 @@ struct BatchArgs {
     const uint32_t* block_scene;  // scene of each preprocess block
     uint32_t nblocks, nviews, nitems;
     const EntityMats* mats;
-    uint64_t* scan_desc;          // 2 x ceil(nitems / 1024) look-back descriptors, zero on entry: the
+    uint64_t* scan_desc;          // 2 x ceil(nitems / kEmitBlockItems) look-back descriptors, zero on entry: the
                                   // emission's, then the item offsets
     uint32_t* counters;           // kCtr* block
     uint32_t cap;                 // pair capacity
@@ -233,7 +233,7 @@ void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, u
 void launch_batch_items(const BatchArgs& a, hipStream_t s);
 void launch_batch_emit(const BatchArgs& a, hipStream_t s);
 void launch_batch_item_offsets(const BatchArgs& a, hipStream_t s);
-constexpr uint32_t kEmitBlockItems = 1024;  // sorted items per emission partition (k_batch_emit)
+constexpr uint32_t kEmitBlockItems = 2048;  // sorted items per emission partition (k_batch_emit)
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s);
