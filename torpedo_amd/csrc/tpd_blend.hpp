@@ -39,7 +39,6 @@ struct BlendLds {
 
 struct BlendBwdLds {
     BlendLds s;
-    uint32_t g[kTilePixels];                // splat index of each staged slot
     double acc[kTilePixels][kSplatGrads];   // per staged splat: the tile's gradient sums (see blend_bwd_tile)
     int32_t last;
 };
@@ -100,9 +99,7 @@ __device__ __forceinline__ StagedSplat load_staged(const Splat* __restrict__ spl
     return t;
 }
 
-template <bool STORE_G>
-__device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_g, const TileView& tv,
-                                                       const StagedSplat& in) {
+__device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, const TileView& tv, const StagedSplat& in) {
     const uint32_t tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = __lane_id();
     {
         constexpr uint32_t s2 = kSentinel | (kSentinel << 16);
@@ -118,14 +115,12 @@ __device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_
     }
     uint32_t mask = 0;
     if (in.valid) {
-        const uint32_t g = in.g;
         const Splat& sp = in.sp;
         const float4 p = sp.xyzr;
         const float4 c = sp.conop;
         s.a[tid] = make_float4(p.x, p.y, c.x, c.y);
         s.b[tid] = make_float4(c.z, sp.aux.x, c.w, 0.0f);
         s.c[tid] = sp.rgb;
-        if (STORE_G) s_g[tid] = g;
         const uint32_t reach = sub_block_mask(p.x, p.y, c.x, c.y, c.z,
                                               SubBlockReach{sp.aux.y, sp.aux.z, sp.aux.w, sp.rgb.w},
                                               (float)(tv.tx * kTile), (float)(tv.ty * kTile));
@@ -171,14 +166,6 @@ __device__ __forceinline__ uint32_t blend_stage_loaded(BlendLds& s, uint32_t* s_
     return walk;
 }
 
-template <bool STORE_G>
-__device__ __forceinline__ uint32_t blend_stage(BlendLds& s, uint32_t* s_g, const TileView& tv, uint32_t base,
-                                                uint32_t end, const uint32_t* __restrict__ vals,
-                                                const Splat* __restrict__ splats) {
-    const uint32_t k = base + threadIdx.x;
-    return blend_stage_loaded<STORE_G>(s, s_g, tv, load_staged(splats, load_value(vals, k, end), k, end));
-}
-
 // Forward composite of one tile.  Per pixel the operations and their order are the reference's;
 // the shortcuts around them change no pixel:
 //   - the sub-block lists of blend_stage drop only splats no pixel of the sub-block can take;
@@ -221,7 +208,7 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
         __syncthreads();
         const uint4 done = *reinterpret_cast<const uint4*>(s.done);
         if ((done.x & done.y & done.z & done.w) != 0u) break;
-        const uint32_t walk = blend_stage_loaded<false>(s, nullptr, tv, next);
+        const uint32_t walk = blend_stage_loaded(s, tv, next);
         next = load_staged(splats, g_after, base + kTilePixels + tid, range.y);
         g_after = load_value(vals, base + 2 * kTilePixels + tid, range.y);
         if (live == 0) continue;
@@ -289,19 +276,26 @@ __device__ __forceinline__ void blend_tile(BlendLds& s, const TileView& tv, uint
     }
 }
 
-// Stores the gradient sums of the pair (this tile, splat g) at its pre-sort position; zeros when
-// out is null.
-__device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t g, const Splat* __restrict__ splats,
-                                                   const uint32_t* __restrict__ offsets, float* __restrict__ partials,
-                                                   const float* out) {
-    const float4 p = splats[g].xyzr;
+// Pre-sort position of the pair (this tile, splat g): g's first pair (offset) + the row-major rank of
+// the tile in g's rectangle (from its record's position and radius).
+__device__ __forceinline__ size_t pair_position(const TileView& tv, const float4& xyzr, uint32_t offset) {
     int32_t x0, y0, x1, y1;
-    bounding_rect(p.x, p.y, p.w, tv.gx, tv.gy, x0, y0, x1, y1);
-    const size_t pos = (size_t)offsets[g] + (size_t)((int32_t)tv.ty - y0) * (size_t)(x1 - x0) +
-                       (size_t)((int32_t)tv.tx - x0);
+    bounding_rect(xyzr.x, xyzr.y, xyzr.w, tv.gx, tv.gy, x0, y0, x1, y1);
+    return (size_t)offset + (size_t)((int32_t)tv.ty - y0) * (size_t)(x1 - x0) + (size_t)((int32_t)tv.tx - x0);
+}
+
+// Stores the gradient sums of a pair at its pre-sort position; zeros when out is null.
+__device__ __forceinline__ void store_partial_at(size_t pos, float* __restrict__ partials, const float* out) {
     float* dst = partials + pos * kSplatGrads;
 #pragma unroll
     for (uint32_t q = 0; q < kSplatGrads; ++q) dst[q] = out ? out[q] : 0.0f;
+}
+
+// The same for the pair (this tile, splat g), its record and offset read here.
+__device__ __forceinline__ void store_pair_partial(const TileView& tv, uint32_t g, const Splat* __restrict__ splats,
+                                                   const uint32_t* __restrict__ offsets, float* __restrict__ partials,
+                                                   const float* out) {
+    store_partial_at(pair_position(tv, splats[g].xyzr, offsets[g]), partials, out);
 }
 
 // Backward of one tile.  The forward frame stored each pixel's final transmittance and last
@@ -363,10 +357,31 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
     float acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f;  // colour accumulated behind the current splat
     float last_alpha = 0.0f, last_r = 0.0f, last_g = 0.0f, last_b = 0.0f;
     const uint32_t first_chunk = ((uint32_t)block_last - range.x) / kTilePixels;
+    // Loads a chunk ahead, as the forward does: while chunk c is walked, the record and the offset of
+    // this thread's splat of chunk c - 1 and the value of chunk c - 2 are in flight.  The offset gives
+    // the pair's pre-sort position once the record is there, so the partial sums of the thread's slot
+    // are stored after the walk without reading anything more.
+    const auto value_of = [&](int32_t c) -> uint32_t {
+        return c >= 0 ? load_value(vals, range.x + (uint32_t)c * kTilePixels + tid, range.y) : 0u;
+    };
+    const auto offset_of = [&](int32_t c, uint32_t g) -> uint32_t {
+        return c >= 0 && range.x + (uint32_t)c * kTilePixels + tid < range.y ? offsets[g] : 0u;
+    };
+    uint32_t g_next = value_of((int32_t)first_chunk);
+    StagedSplat next = load_staged(splats, g_next, range.x + first_chunk * kTilePixels + tid, range.y);
+    uint32_t off_next = offset_of((int32_t)first_chunk, g_next);
+    uint32_t g_after = value_of((int32_t)first_chunk - 1);
     for (int32_t chunk = (int32_t)first_chunk; chunk >= 0; --chunk) {
         const uint32_t base = range.x + (uint32_t)chunk * kTilePixels;
+        const StagedSplat cur = next;
+        const size_t pos = cur.valid ? pair_position(tv, cur.sp.xyzr, off_next) : 0;
         __syncthreads();
-        const uint32_t walk = blend_stage<true>(s, L.g, tv, base, range.y, vals, splats);
+        const uint32_t walk = blend_stage_loaded(s, tv, cur);
+        if (chunk > 0) {
+            next = load_staged(splats, g_after, base - kTilePixels + tid, range.y);
+            off_next = offset_of(chunk - 1, g_after);
+            g_after = value_of(chunk - 2);
+        }
         for (int32_t i = (int32_t)walk - 1; i >= 0; --i) {
             const uint32_t off = list[i];
             const uint32_t j = off / kSlotBytes;  // 256 = the sentinel, never active
@@ -389,7 +404,10 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             for (uint32_t q = 0; q < kG; ++q) v[q] = 0.0f;
             if (upd) {
                 const float4 C = slot4(s.c, off);
-                T = T / (1.0f - alpha);  // transmittance in front of this splat
+                // transmittance in front of this splat: 1 - alpha >= 0.01, so the hardware reciprocal
+                // (about 1 ulp) stands in for the correctly rounded division (the gradients are
+                // checked against float64, not bit for bit; plain and batched frames run this same code)
+                T = T * __builtin_amdgcn_rcpf(1.0f - alpha);
                 acc_r = last_alpha * last_r + (1.0f - last_alpha) * acc_r;
                 acc_g = last_alpha * last_g + (1.0f - last_alpha) * acc_g;
                 acc_b = last_alpha * last_b + (1.0f - last_alpha) * acc_b;
@@ -420,14 +438,14 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             }
         }
         __syncthreads();
-        if (base + tid < range.y) {
+        if (cur.valid) {
             float out[kG];
 #pragma unroll
             for (uint32_t q = 0; q < kG; ++q) {
                 out[q] = (float)L.acc[tid][q];
                 L.acc[tid][q] = 0.0;
             }
-            store_pair_partial(tv, L.g[tid], splats, offsets, partials, out);
+            store_partial_at(pos, partials, out);
         }
     }
 }
