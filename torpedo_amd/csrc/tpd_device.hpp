@@ -389,8 +389,6 @@ __device__ __forceinline__ void store_splat(Splat* dst, bool vis, const Projecte
 
 // ------------------------------------------------------------------ wave64 helpers
 
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
 // Wave-wide vote: bit l set when lane l's predicate holds (a scalar AND with exec).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
@@ -440,31 +438,6 @@ __device__ __forceinline__ float row_sum16(float v) {
     v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
     x = __builtin_bit_cast(int, v);
     v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));
-    return v;
-}
-
-// Sum over the 64 lanes with DPP moves only (no LDS permutes); the total lands in lane 63.
-// quad_perm [1,0,3,2] and [2,3,0,1] -> quad sums, row_half_mirror -> 8-lane sums, row_mirror ->
-// row (16-lane) sums, row_bcast15 into rows 1 and 3 -> 32-lane sums, row_bcast31 into rows 2-3.
-__device__ __forceinline__ float wave_sum_to_lane63(float v) {
-    int x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
-    x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
-    x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
-    x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));
-    x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false));
-    x = __builtin_bit_cast(int, v);
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false));
-    return v;
-}
-
-__device__ __forceinline__ float wave_sum_f32(float v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
 }
 
