@@ -395,13 +395,27 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
                                                                     const uint32_t* __restrict__ block_scene,
                                                                     const EntityMats* __restrict__ mats,
                                                                     const BatchGradsDev* __restrict__ grads) {
+    // Each thread's record is read from HBM once and kept in LDS for the views of its scene: read
+    // again per view, the records of the resident blocks (about 6 MB per XCD) outgrow the XCD's L2
+    // and every view fetched them anew.  A stride of 61 floats keeps the lanes' reads of one field
+    // on distinct banks; each thread reads only its own record, so no barrier is needed.
+    constexpr uint32_t kRecStride = 61;
+    __shared__ float s_rec[kPreBlock * kRecStride];
     if (a.counters[kCtrOverflow]) return;  // gradients are left untouched; the step is run again
     const uint32_t b = blockIdx.x;
     const uint32_t si = block_scene[b];
     const BatchSceneDev& sc = scenes[si];
     const uint32_t g = (b - sc.first_block) * kPreBlock + threadIdx.x;
     if (g >= sc.n) return;
-    const float* rec = sc.gauss + (size_t)g * 60;
+    float* rec = s_rec + threadIdx.x * kRecStride;
+    {
+        const float4* src = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
+#pragma unroll
+        for (int q = 0; q < 15; ++q) {
+            const float4 v = src[q];
+            rec[4 * q + 0] = v.x; rec[4 * q + 1] = v.y; rec[4 * q + 2] = v.z; rec[4 * q + 3] = v.w;
+        }
+    }
     const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
     // The same FP32 operations as a plain backward per view accumulated into the gradient buffers:
     // every log-scale, quaternion, opacity and SH entry receives one term per view, added to the
