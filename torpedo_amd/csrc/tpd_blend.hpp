@@ -382,22 +382,12 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
             off_next = offset_of(chunk - 1, g_after);
             g_after = value_of(chunk - 2);
         }
-        for (int32_t i = (int32_t)walk - 1; i >= 0; --i) {
-            const uint32_t off = list[i];
-            const uint32_t j = off / kSlotBytes;  // 256 = the sentinel, never active
-            const int32_t kk = (int32_t)(base + j);
-            const uint64_t mine = ballot(kk <= last);
-            if (mine == 0) continue;
-            const float4 A = slot4(s.a, off);
-            const float4 B = slot4(s.b, off);
-            const float dx = A.x - fx, dy = A.y - fy;
-            const float power = -0.5f * ((A.z * dx) * dx + (B.x * dy) * dy) - (A.w * dx) * dy;
-            const uint64_t act = ballot(power <= 0.0f) & ballot(!(power < B.y)) & mine;
-            if (act == 0) continue;
-            const float G = soft_expf_inrange(power);
-            const float raw = B.z * G;
-            const float alpha = raw < 0.99f ? raw : 0.99f;
-            const uint64_t updm = act & ballot(!(alpha < 1.0f / 255.0f));
+        // Two list entries per step, back to front: both splats' powers, exponentials and alphas
+        // are formed first (none depends on the transmittance), then the two updates run in order.
+        // A step past the list's start reads the sentinel entry, which no lane takes.
+        const auto entry_update = [&](uint32_t off, const float4& A, const float4& B, float dx, float dy,
+                                      float G, float raw, float alpha, uint64_t updm) {
+            const uint32_t j = off / kSlotBytes;
             const bool upd = lane_on(updm);
             float v[kG];
 #pragma unroll
@@ -428,14 +418,34 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                     v[4] = dL_dpower * (-0.5f * dy * dy);
                 }
             }
-            if (updm != 0) {
-                // row sums by DPP, then one LDS atomic per row of 16 lanes
+            // row sums by DPP, then one LDS atomic per row of 16 lanes
 #pragma unroll
-                for (uint32_t q = 0; q < kG; ++q) {
-                    const float sum = row_sum16(v[q]);
-                    if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], (double)sum);
-                }
+            for (uint32_t q = 0; q < kG; ++q) {
+                const float sum = row_sum16(v[q]);
+                if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], (double)sum);
             }
+        };
+        for (int32_t i = (int32_t)walk - 1; i >= 0; i -= 2) {
+            const uint32_t o1 = list[i], o2 = i >= 1 ? list[i - 1] : kSentinel;
+            // lanes whose last blended splat lies at or behind the entry (256 = the sentinel)
+            const uint64_t mine1 = ballot((int32_t)(base + o1 / kSlotBytes) <= last);
+            const uint64_t mine2 = ballot((int32_t)(base + o2 / kSlotBytes) <= last);
+            if ((mine1 | mine2) == 0) continue;
+            const float4 A1 = slot4(s.a, o1), B1 = slot4(s.b, o1);
+            const float4 A2 = slot4(s.a, o2), B2 = slot4(s.b, o2);
+            const float dx1 = A1.x - fx, dy1 = A1.y - fy, dx2 = A2.x - fx, dy2 = A2.y - fy;
+            const float p1 = -0.5f * ((A1.z * dx1) * dx1 + (B1.x * dy1) * dy1) - (A1.w * dx1) * dy1;
+            const float p2 = -0.5f * ((A2.z * dx2) * dx2 + (B2.x * dy2) * dy2) - (A2.w * dx2) * dy2;
+            const uint64_t act1 = ballot(p1 <= 0.0f) & ballot(!(p1 < B1.y)) & mine1;
+            const uint64_t act2 = ballot(p2 <= 0.0f) & ballot(!(p2 < B2.y)) & mine2;
+            if ((act1 | act2) == 0) continue;
+            const float G1 = soft_expf_inrange(p1), G2 = soft_expf_inrange(p2);
+            const float raw1 = B1.z * G1, raw2 = B2.z * G2;
+            const float alpha1 = raw1 < 0.99f ? raw1 : 0.99f, alpha2 = raw2 < 0.99f ? raw2 : 0.99f;
+            const uint64_t updm1 = act1 & ballot(!(alpha1 < 1.0f / 255.0f));
+            const uint64_t updm2 = act2 & ballot(!(alpha2 < 1.0f / 255.0f));
+            if (updm1 != 0) entry_update(o1, A1, B1, dx1, dy1, G1, raw1, alpha1, updm1);
+            if (updm2 != 0) entry_update(o2, A2, B2, dx2, dy2, G2, raw2, alpha2, updm2);
         }
         __syncthreads();
         if (cur.valid) {
