@@ -380,6 +380,18 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     a.item_vals = b->d_ivals;
     a.lo_bits = b->lo_bits();
     a.lo_base = b->lo_base();
+    // the item sort's digit counts come from the item kernel when its passes fit the LDS histograms
+    const uint32_t ipasses = sort_shape_passes(b->item_shape, a.lo_bits);
+    if (b->nitems && ipasses <= kMaxCountedPasses) {
+        const uint32_t ibits = sort_shape_bits(b->item_shape);
+        const uint32_t iparts = (b->nitems + sort_shape_keys(b->item_shape) - 1) / sort_shape_keys(b->item_shape);
+        a.hist_counts = b->ws.partials;
+        a.hist_desc0 = b->ws.desc;
+        a.hist_tickets = b->ws.tickets;
+        a.hist_bits = ibits;
+        a.hist_passes = ipasses;
+        a.hist_desc0_words = iparts << ibits;
+    }
     a.keys = b->d_keys;
     a.vals = b->d_vals;
     a.ranges = b->d_ranges;
@@ -426,7 +438,8 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
         const uint32_t pair_shape = b->ws.shape;
         b->ws.shape = b->item_shape;
         items_alt = launch_sort_pairs(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
-                                      b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr);
+                                      b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr, 32u, nullptr, 0u,
+                                      a.hist_counts != nullptr);
         b->ws.shape = pair_shape;
     } else if (events)
         hipEventRecord(b->ev[9], s);

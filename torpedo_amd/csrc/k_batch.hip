@@ -51,23 +51,13 @@ __device__ __forceinline__ bool project_in_view(const float* rec, const EntityMa
 
 // ------------------------------------------------------------------ items
 
-// Per (Gaussian, view) item: project, write the splat record of a visible item, its emission
-// record (and tile count for the backward) and its depth key and index for the item sort.
-// Each block reads its 256 Gaussian records once, for all views of their scene; the threads do
-// not communicate, so no barrier is needed.  The block also clears its share of the tile ranges.
-__global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
-                                                           const BatchViewDev* __restrict__ views,
-                                                           const uint32_t* __restrict__ scene_views,
-                                                           const uint32_t* __restrict__ block_scene,
-                                                           const EntityMats* __restrict__ mats) {
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
-        a.ranges[t] = make_uint2(0u, 0u);
-    const uint32_t b = blockIdx.x;
-    if (b >= a.nblocks) return;  // the one block of a batch without Gaussians only clears
-    const BatchSceneDev& sc = scenes[block_scene[b]];
-    const uint32_t g = (b - sc.first_block) * kPreBlock + tid;
-    if (g >= sc.n) return;
+// One Gaussian's items, for every view of its scene (the body of k_batch_items); with h, the
+// digits of each item key are counted in the block's LDS histograms.
+__device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const BatchSceneDev& sc,
+                                                  const BatchViewDev* __restrict__ views,
+                                                  const uint32_t* __restrict__ scene_views,
+                                                  const EntityMats* __restrict__ mats, uint32_t g,
+                                                  uint32_t (*h)[kMaxRadix]) {
     const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
     const uint32_t culled_key = a.lo_bits >= 32 ? 0xFFFFFFFFu : (1u << a.lo_bits) - 1u;
 
@@ -99,8 +89,52 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, co
         a.item_keys[item] = key;
         a.item_vals[item] = item;
         if (a.tiles) a.tiles[item] = p.tiles;
+        if (h) {
+            const uint32_t k32 = (uint32_t)key;  // the depth key: the bits the item sort orders by
+            const uint32_t dmask = (1u << a.hist_bits) - 1u;
+            for (uint32_t q = 0; q < a.hist_passes; ++q) atomicAdd(&h[q][(k32 >> (q * a.hist_bits)) & dmask], 1u);
+        }
     }
-    if (ballot(depth_out) != 0 && __lane_id() == 0) atomicOr(&a.counters[kCtrDepthOut], 1u);
+    if (depth_out) atomicOr(&a.counters[kCtrDepthOut], 1u);
+}
+
+// Per (Gaussian, view) item: project, write the splat record of a visible item, its emission
+// record (and tile count for the backward) and its depth key and index for the item sort.
+// Each block reads its 256 Gaussian records once, for all views of their scene.  With
+// a.hist_counts it also counts the item keys' digits for the item sort in LDS and adds its counts
+// into the sort's digit counts (the sort's own histogram pass, a second read of every key, is then
+// left out), and clears the first pass's look-back descriptors and the tickets.  The block also
+// clears its share of the tile ranges.
+__global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
+                                                           const BatchViewDev* __restrict__ views,
+                                                           const uint32_t* __restrict__ scene_views,
+                                                           const uint32_t* __restrict__ block_scene,
+                                                           const EntityMats* __restrict__ mats) {
+    __shared__ uint32_t h[kMaxCountedPasses][kMaxRadix];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
+        a.ranges[t] = make_uint2(0u, 0u);
+    const uint32_t b = blockIdx.x;
+    if (b >= a.nblocks) return;  // the one block of a batch without Gaussians only clears
+    const bool counting = a.hist_counts != nullptr;  // block-uniform
+    if (counting) {
+        for (uint32_t x = blockIdx.x * kPreBlock + tid; x < a.hist_desc0_words; x += gridDim.x * kPreBlock)
+            a.hist_desc0[x] = 0u;
+        if (b == 0 && tid < kMaxPasses) a.hist_tickets[tid] = 0u;
+        for (uint32_t x = tid; x < kMaxCountedPasses * kMaxRadix; x += kPreBlock) (&h[0][0])[x] = 0u;
+        __syncthreads();
+    }
+    const BatchSceneDev& sc = scenes[block_scene[b]];
+    const uint32_t g = (b - sc.first_block) * kPreBlock + tid;
+    if (g < sc.n) items_of_gaussian(a, sc, views, scene_views, mats, g, counting ? h : nullptr);
+    if (!counting) return;
+    __syncthreads();
+    const uint32_t R = 1u << a.hist_bits;
+    uint32_t* counts = a.hist_counts + (b % kCountCopies) * (kMaxPasses * kMaxRadix);
+    for (uint32_t x = tid; x < a.hist_passes * R; x += kPreBlock) {
+        const uint32_t v = h[x / R][x % R];
+        if (v) atomicAdd(&counts[x], v);  // x = pass * R + digit, the layout of the sort's counts
+    }
 }
 
 void launch_batch_items(const BatchArgs& a, hipStream_t s) {

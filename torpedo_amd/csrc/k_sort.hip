@@ -446,7 +446,7 @@ static uint32_t hist_blocks(uint32_t n_max) {
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop, uint32_t lo_base) {
+                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop, uint32_t lo_base, bool counted) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -455,8 +455,14 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
 
     const uint32_t hb = hist_blocks(n_max);
-    if (sh.bits == 8) launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
-    else launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
+    if (counted) {
+        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+    } else if (sh.bits == 8) {
+        launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
+    } else {
+        launch_hist<9>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
+    }
     if (after_hist) hipEventRecord(after_hist, s);
     // With narrow_drop, the first pass whose digit starts at or above key_bits - 32 writes 32-bit
     // keys (the bits from its digit up) and the later passes move 8 instead of 12 bytes per pair.

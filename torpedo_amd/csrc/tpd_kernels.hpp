@@ -178,6 +178,14 @@ struct BatchArgs {
                                   // record as inline_rect encodes it
     uint32_t* item_vals;          // per item: the item itself
     uint32_t lo_bits, lo_base;
+    // the item sort's digit histograms, counted by the item kernel (the sort then skips its own):
+    // counts as k_sort_hist leaves them (copy blockIdx mod kCountCopies), digit width and passes of
+    // the item sort's shape; the first pass's look-back descriptors (desc0_words) and the tickets
+    // are cleared here too
+    uint32_t* hist_counts;
+    uint32_t* hist_desc0;
+    uint32_t* hist_tickets;
+    uint32_t hist_bits, hist_passes, hist_desc0_words;
     const uint64_t* sorted_keys64;  // emission input: the item keys in depth order ...
     const uint32_t* sorted_items;   // ... and their items
     uint32_t* keys;               // pairs: global tile
@@ -279,7 +287,10 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits);  // passes for ke
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
                        hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
-                       uint32_t lo_base = 0);
+                       uint32_t lo_base = 0, bool counted = false);
+// With counted, the digit counts are already in ws.partials and the first pass's descriptors and the
+// tickets already cleared (by the kernel that made the keys): the histogram kernel is not launched.
+constexpr uint32_t kMaxCountedPasses = 4;  // passes whose counts the item kernel can make
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
 // moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
