@@ -359,6 +359,8 @@ static int ensure_partials(BatchState* b, size_t pairs) {
 
 // ------------------------------------------------------------------ the recorded frame
 
+static uint32_t tile_key_bits(const BatchState* b) { return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1); }
+
 static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     BatchArgs a{};
     a.scenes = b->d_scenes;
@@ -391,6 +393,16 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
         a.hist_bits = ibits;
         a.hist_passes = ipasses;
         a.hist_desc0_words = iparts << ibits;
+        // and the pair sort's from the items' tile rectangles, when its passes fit too
+        const uint32_t pshape = b->ws.shape, pbits = sort_shape_bits(pshape);
+        const uint32_t ppasses = sort_shape_passes(pshape, tile_key_bits(b));
+        if (b->cap && ppasses >= 1 && ppasses <= kMaxCountedPairPasses) {
+            const uint32_t pparts = (b->cap + sort_shape_keys(pshape) - 1) / sort_shape_keys(pshape);
+            a.pair_counts = b->ws.partials + kPairCountRow;
+            a.pair_bits = pbits;
+            a.pair_passes = ppasses;
+            a.pair_desc0_words = pparts << pbits;
+        }
     }
     a.keys = b->d_keys;
     a.vals = b->d_vals;
@@ -399,7 +411,6 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
     return a;
 }
 
-static uint32_t tile_key_bits(const BatchState* b) { return bit_length(b->ntiles > 1 ? b->ntiles - 1 : 1); }
 
 static BatchBlendArgs blend_args(BatchState* b) {
     BatchBlendArgs a{};
@@ -451,9 +462,11 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     if (events) hipEventRecord(b->ev[3], s);
     const uint32_t key_bits = tile_key_bits(b);
     b->pair_passes = sort_shape_passes(b->ws.shape, key_bits);
+    // kCtrSortPairs is P, or 0 after an overflow: the counted digits then cover pairs that were
+    // never written, and the sort and the ranges must not run on them
     const bool in_alt = launch_sort_pairs32(b->ws, b->d_keys, b->d_vals, b->d_keys_alt, b->d_vals_alt,
-                                            &b->d_counters[kCtrPairs], b->cap, key_bits, s,
-                                            events ? b->ev[10] : nullptr);
+                                            &b->d_counters[kCtrSortPairs], b->cap, key_bits, s,
+                                            events ? b->ev[10] : nullptr, a.pair_counts != nullptr);
     b->sorted_keys = in_alt ? b->d_keys_alt : b->d_keys;
     b->sorted_vals = in_alt ? b->d_vals_alt : b->d_vals;
     if (events) hipEventRecord(b->ev[4], s);
@@ -462,7 +475,7 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
     launch_ranges(b->sorted_keys, b->d_counters, b->cap, b->ntiles, b->d_ranges, b->d_scan_desc,
                   2 * ((b->nitems + kEmitBlockItems - 1) / kEmitBlockItems), &b->d_counters[kCtrScanTicket], s, b->cap,
                   b->lo_bits(),
-                  b->lo_bits());
+                  b->lo_bits(), kCtrSortPairs);
     launch_tile_order(b->d_ranges, b->ntiles, b->d_counters, b->d_tile_order, s);
     if (events) hipEventRecord(b->ev[5], s);
     BatchBlendArgs ba = blend_args(b);

@@ -51,13 +51,59 @@ __device__ __forceinline__ bool project_in_view(const float* rec, const EntityMa
 
 // ------------------------------------------------------------------ items
 
-// One Gaussian's items, for every view of its scene (the body of k_batch_items); with h, the
-// digits of each item key are counted in the block's LDS histograms.
+// The item kernel's digit histograms of the two sorts that follow it.
+struct ItemCountLds {
+    uint32_t item[kMaxCountedPasses][kMaxRadix];      // item key (depth key) digits, per pass
+    uint32_t pair[kMaxCountedPairPasses][kMaxRadix];  // pair key (global tile) digits, per pass
+    uint32_t pair_diff[kMaxRadix];  // pass 0: difference array (modulo 2^32) of the digit counts
+    uint32_t pair_all;              // pass 0: added to every digit
+    uint32_t wave_sum[kPreBlock / kWave];
+};
+
+// Counts the pair keys of a tile rectangle: tiles tile0 + r * gx + c, r < h, c < w (w, h >= 1).
+// Row r's w tiles are consecutive, so their pass-0 digits are the circular run of w digits from
+// d0 = first tile mod R: +1 at d0 and -1 at (d0 + w) mod R in the difference array, whose prefix
+// sums then fall one short on every digit for each time the run passes digit R - 1, i.e.
+// (d0 + w) >> bits times; that number goes into all0 (the thread's share of pair_all).  A higher
+// pass adds the whole rectangle to one digit when its first and last tile share that digit's
+// prefix, else each row's segments.
+__device__ __forceinline__ void count_pair_digits(ItemCountLds& L, uint32_t& all0, uint32_t tile0, uint32_t gx,
+                                                  uint32_t w, uint32_t h, uint32_t bits, uint32_t passes) {
+    const uint32_t dmask = (1u << bits) - 1u;
+    const uint32_t last = tile0 + (h - 1u) * gx + (w - 1u);
+    for (uint32_t q = 1; q < passes; ++q) {
+        const uint32_t sh = q * bits;
+        if ((tile0 >> sh) == (last >> sh)) {
+            atomicAdd(&L.pair[q][(tile0 >> sh) & dmask], w * h);
+            continue;
+        }
+        for (uint32_t r = 0; r < h; ++r) {
+            uint32_t lo = tile0 + r * gx;
+            const uint32_t hi = lo + (w - 1u);
+            for (uint32_t seg = lo >> sh; seg <= (hi >> sh); ++seg) {
+                const uint32_t end = min(hi, ((seg + 1u) << sh) - 1u);
+                atomicAdd(&L.pair[q][seg & dmask], end - lo + 1u);
+                lo = end + 1u;
+            }
+        }
+    }
+    uint32_t t0 = tile0;
+    for (uint32_t r = 0; r < h; ++r, t0 += gx) {
+        const uint32_t d0 = t0 & dmask;
+        atomicAdd(&L.pair_diff[d0], 1u);
+        atomicSub(&L.pair_diff[(d0 + w) & dmask], 1u);
+        all0 += (d0 + w) >> bits;
+    }
+}
+
+// One Gaussian's items, for every view of its scene (the body of k_batch_items); with L, the
+// digits of each item key (and, with a.pair_counts, of each visible item's pair keys) are counted
+// in the block's LDS histograms.
 __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const BatchSceneDev& sc,
                                                   const BatchViewDev* __restrict__ views,
                                                   const uint32_t* __restrict__ scene_views,
                                                   const EntityMats* __restrict__ mats, uint32_t g,
-                                                  uint32_t (*h)[kMaxRadix]) {
+                                                  ItemCountLds* L, uint32_t& all0) {
     const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
     const uint32_t culled_key = a.lo_bits >= 32 ? 0xFFFFFFFFu : (1u << a.lo_bits) - 1u;
 
@@ -85,14 +131,16 @@ __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const Batc
             if (r32 == kRectLookup) a.rects[item] = pack_emit_rect(tile0, vw.gx, w, h);
             key = ((uint64_t)r32 << 32) | ((as_uint(p.vz) - a.lo_base) & culled_key);
             depth_out |= depth_outside_window(p.vz);
+            if (L && a.pair_counts) count_pair_digits(*L, all0, tile0, vw.gx, w, h, a.pair_bits, a.pair_passes);
         }
         a.item_keys[item] = key;
         a.item_vals[item] = item;
         if (a.tiles) a.tiles[item] = p.tiles;
-        if (h) {
+        if (L) {
             const uint32_t k32 = (uint32_t)key;  // the depth key: the bits the item sort orders by
             const uint32_t dmask = (1u << a.hist_bits) - 1u;
-            for (uint32_t q = 0; q < a.hist_passes; ++q) atomicAdd(&h[q][(k32 >> (q * a.hist_bits)) & dmask], 1u);
+            for (uint32_t q = 0; q < a.hist_passes; ++q)
+                atomicAdd(&L->item[q][(k32 >> (q * a.hist_bits)) & dmask], 1u);
         }
     }
     if (depth_out) atomicOr(&a.counters[kCtrDepthOut], 1u);
@@ -103,37 +151,76 @@ __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const Batc
 // Each block reads its 256 Gaussian records once, for all views of their scene.  With
 // a.hist_counts it also counts the item keys' digits for the item sort in LDS and adds its counts
 // into the sort's digit counts (the sort's own histogram pass, a second read of every key, is then
-// left out), and clears the first pass's look-back descriptors and the tickets.  The block also
-// clears its share of the tile ranges.
+// left out), and clears the first pass's look-back descriptors and the tickets; with
+// a.pair_counts likewise the digits of the pair keys the emission will write (every tile of each
+// visible item's rectangle), for the pair sort.  The block also clears its share of the tile
+// ranges, and block 0 the pair sort's count, which the emission sets.
 __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, const BatchSceneDev* __restrict__ scenes,
                                                            const BatchViewDev* __restrict__ views,
                                                            const uint32_t* __restrict__ scene_views,
                                                            const uint32_t* __restrict__ block_scene,
                                                            const EntityMats* __restrict__ mats) {
-    __shared__ uint32_t h[kMaxCountedPasses][kMaxRadix];
-    const uint32_t tid = threadIdx.x;
+    __shared__ ItemCountLds L;
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave;
     for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
         a.ranges[t] = make_uint2(0u, 0u);
+    if (blockIdx.x == 0 && tid == 0) a.counters[kCtrSortPairs] = 0u;
     const uint32_t b = blockIdx.x;
     if (b >= a.nblocks) return;  // the one block of a batch without Gaussians only clears
     const bool counting = a.hist_counts != nullptr;  // block-uniform
+    const bool pairs = counting && a.pair_counts != nullptr;
     if (counting) {
         for (uint32_t x = blockIdx.x * kPreBlock + tid; x < a.hist_desc0_words; x += gridDim.x * kPreBlock)
             a.hist_desc0[x] = 0u;
         if (b == 0 && tid < kMaxPasses) a.hist_tickets[tid] = 0u;
-        for (uint32_t x = tid; x < kMaxCountedPasses * kMaxRadix; x += kPreBlock) (&h[0][0])[x] = 0u;
+        for (uint32_t x = tid; x < kMaxCountedPasses * kMaxRadix; x += kPreBlock) (&L.item[0][0])[x] = 0u;
+        if (pairs) {
+            for (uint32_t x = tid; x < kMaxCountedPairPasses * kMaxRadix; x += kPreBlock) (&L.pair[0][0])[x] = 0u;
+            for (uint32_t x = tid; x < kMaxRadix; x += kPreBlock) L.pair_diff[x] = 0u;
+            if (tid == 0) L.pair_all = 0u;
+        }
         __syncthreads();
     }
     const BatchSceneDev& sc = scenes[block_scene[b]];
     const uint32_t g = (b - sc.first_block) * kPreBlock + tid;
-    if (g < sc.n) items_of_gaussian(a, sc, views, scene_views, mats, g, counting ? h : nullptr);
+    uint32_t all0 = 0;
+    if (g < sc.n) items_of_gaussian(a, sc, views, scene_views, mats, g, counting ? &L : nullptr, all0);
     if (!counting) return;
+    if (pairs) {
+        const uint32_t s_all = wave_incl_scan(all0);
+        if (lane == kWave - 1 && s_all) atomicAdd(&L.pair_all, s_all);
+    }
     __syncthreads();
     const uint32_t R = 1u << a.hist_bits;
     uint32_t* counts = a.hist_counts + (b % kCountCopies) * (kMaxPasses * kMaxRadix);
     for (uint32_t x = tid; x < a.hist_passes * R; x += kPreBlock) {
-        const uint32_t v = h[x / R][x % R];
+        const uint32_t v = L.item[x / R][x % R];
         if (v) atomicAdd(&counts[x], v);  // x = pass * R + digit, the layout of the sort's counts
+    }
+    if (!pairs) return;
+    // pass 0 of the pair keys: pair_all plus the inclusive prefix sum of the difference array
+    // (modulo 2^32, where its negative entries live); each thread takes R / 256 consecutive
+    // digits (R = 256 or 512)
+    const uint32_t PR = 1u << a.pair_bits, per = PR / kPreBlock;
+    uint32_t dv[2] = {0u, 0u}, local = 0;
+    for (uint32_t i = 0; i < per; ++i) {
+        dv[i] = L.pair_diff[tid * per + i];
+        local += dv[i];
+    }
+    const uint32_t incl = wave_incl_scan(local);
+    if (lane == kWave - 1) L.wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - local + L.pair_all;
+    for (uint32_t w = 0; w < wave; ++w) run += L.wave_sum[w];
+    for (uint32_t i = 0; i < per; ++i) {
+        run += dv[i];
+        L.pair[0][tid * per + i] = run;
+    }
+    __syncthreads();
+    uint32_t* pcounts = a.pair_counts + (b % kCountCopies) * (kMaxPasses * kMaxRadix);
+    for (uint32_t x = tid; x < a.pair_passes * PR; x += kPreBlock) {
+        const uint32_t v = L.pair[x / PR][x % PR];
+        if (v) atomicAdd(&pcounts[x], v);
     }
 }
 
@@ -182,6 +269,11 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     uint64_t* const s_rect = L.big.rect;
 
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
+    // the pair sort's counts come from the item kernel: its first pass's descriptors (which the
+    // item sort's last pass may have used) and the tickets are cleared here
+    for (uint32_t x = blockIdx.x * kPreBlock + tid; x < a.pair_desc0_words; x += gridDim.x * kPreBlock)
+        a.hist_desc0[x] = 0u;
+    if (a.pair_desc0_words && blockIdx.x == 0 && tid < kMaxPasses) a.hist_tickets[tid] = 0u;
     if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
     __syncthreads();
     const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);
@@ -258,6 +350,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
                 const uint64_t total = excl + block_total;
                 a.counters[kCtrPairs] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
                 a.counters[kCtrOverflow] = total > a.cap ? 1u : 0u;
+                a.counters[kCtrSortPairs] = total > a.cap ? 0u : (uint32_t)total;
             }
         }
     }

@@ -180,8 +180,8 @@ __device__ __forceinline__ void ranges_of_run(const KT* __restrict__ keys, uint3
 // overflowed, so that every later kernel skips it and the host renders it again on the wider key.
 __global__ void k_ranges(const void* keys, uint32_t* __restrict__ counters, uint32_t cap,
                          uint2* __restrict__ ranges, uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket,
-                         uint32_t key_drop, uint32_t lo_bits) {
-    const uint32_t P = min(counters[kCtrPairs], cap);
+                         uint32_t key_drop, uint32_t lo_bits, uint32_t count_slot) {
+    const uint32_t P = min(counters[count_slot], cap);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nthreads = gridDim.x * blockDim.x;
     for (uint32_t d = gid; d < n_scan_desc; d += nthreads) scan_desc[d] = 0;
@@ -203,14 +203,14 @@ __global__ void k_ranges(const void* keys, uint32_t* __restrict__ counters, uint
 
 void launch_ranges(const void* keys, uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s, uint32_t cap,
-                   uint32_t key_drop, uint32_t lo_bits) {
+                   uint32_t key_drop, uint32_t lo_bits, uint32_t count_slot) {
     (void)ntiles;
     uint32_t work = (P_hint + kRangeRun - 1) / kRangeRun;
     work = work > n_scan_desc ? work : n_scan_desc;
     uint32_t nb = (work + 255) / 256;
     nb = nb < 1 ? 1 : (nb > 8192 ? 8192 : nb);
     hipLaunchKernelGGL(k_ranges, dim3(nb), dim3(256), 0, s, keys, counters, cap, ranges, scan_desc, n_scan_desc, ticket,
-                       key_drop, lo_bits);
+                       key_drop, lo_bits, count_slot);
 }
 
 __global__ void k_full_keys(const uint32_t* __restrict__ keys32, uint32_t tshift, const uint32_t* __restrict__ vals,

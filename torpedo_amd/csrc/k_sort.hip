@@ -502,7 +502,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist) {
+                         hipEvent_t after_hist, bool counted) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -510,8 +510,15 @@ bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint
     const uint32_t parts = (n_max + tile - 1) / tile;
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
     const uint32_t hb = hist_blocks(n_max);
-    if (sh.bits == 8) launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
-    else launch_hist<9>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
+    if (counted) {
+        uint32_t* counts = ws.partials + kPairCountRow;
+        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, counts, ws.offsets);
+        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, counts, ws.offsets);
+    } else if (sh.bits == 8) {
+        launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
+    } else {
+        launch_hist<9>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
+    }
     if (after_hist) hipEventRecord(after_hist, s);
     const uint32_t* kin = keys;
     const uint32_t* vin = vals;

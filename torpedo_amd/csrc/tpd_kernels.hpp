@@ -12,6 +12,8 @@ enum : uint32_t {
     kCtrOverflow = 2,     // 1 when P exceeded the key/value capacity
     kCtrDepthOut = 3,     // non-zero when an emitted depth lies outside the sort's depth window
     kCtrScanTicket2 = 4,  // partition ticket of a second scan in the frame (the batch's item offsets)
+    kCtrSortPairs = 5,    // the batch's pair count for its pair sort and ranges: P, or 0 when P exceeded
+                          // the capacity (that frame is rendered again)
     kCtrCount = 16,
 };
 
@@ -100,12 +102,13 @@ void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s);
 // (compacted key (tile << lo_bits) | (depth - lo_base)) >> drop; key_drop = kKeysWide marks the
 // first form.
 constexpr uint32_t kKeysWide = 0xFFFFFFFFu;
-// Tile ranges of the sorted keys; P = min(counters[kCtrPairs], cap).  Also re-zeroes the scan
+// Tile ranges of the sorted keys; P = min(counters[count_slot], cap).  Also re-zeroes the scan
 // descriptors and the scan ticket for the next frame, and sets kCtrOverflow when a frame sorted
 // on the depth-window key (lo_bits = kDepthWindowBits) has kCtrDepthOut set.
 void launch_ranges(const void* keys, uint32_t* counters, uint32_t P_hint, uint32_t ntiles, uint2* ranges,
                    uint64_t* scan_desc, uint32_t n_scan_desc, uint32_t* ticket, hipStream_t s,
-                   uint32_t cap = 0xFFFFFFFFu, uint32_t key_drop = kKeysWide, uint32_t lo_bits = kDepthWideBits);
+                   uint32_t cap = 0xFFFFFFFFu, uint32_t key_drop = kKeysWide, uint32_t lo_bits = kDepthWideBits,
+                   uint32_t count_slot = kCtrPairs);
 // The 64-bit keys (tile << 32) | floatbits(view depth) of n sorted 32-bit keys, the tile from the
 // key and the depth from the splat of each value (for readers of the sorted pairs).
 void launch_full_keys(const uint32_t* keys32, uint32_t drop, uint32_t lo_bits, const uint32_t* vals,
@@ -186,6 +189,12 @@ struct BatchArgs {
     uint32_t* hist_desc0;
     uint32_t* hist_tickets;
     uint32_t hist_bits, hist_passes, hist_desc0_words;
+    // the pair sort's digit histograms, counted by the item kernel from each visible item's tile
+    // rectangle (needs hist_counts): counts from row kPairCountRow of the sort's digit counts, digit
+    // width and passes of the pair sort's shape; the emission clears the pair sort's first-pass
+    // descriptors (pair_desc0_words, for the pair capacity) and the tickets
+    uint32_t* pair_counts;
+    uint32_t pair_bits, pair_passes, pair_desc0_words;
     const uint64_t* sorted_keys64;  // emission input: the item keys in depth order ...
     const uint32_t* sorted_items;   // ... and their items
     uint32_t* keys;               // pairs: global tile
@@ -292,9 +301,14 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 // tickets already cleared (by the kernel that made the keys): the histogram kernel is not launched.
 constexpr uint32_t kMaxCountedPasses = 4;  // passes whose counts the item kernel can make
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
-// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.
+// moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.  With counted,
+// the digit counts are in ws.partials from word kPairCountRow of each copy on, and the first pass's
+// descriptors and the tickets are already cleared.
 bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                         hipEvent_t after_hist = nullptr);
+                         hipEvent_t after_hist = nullptr, bool counted = false);
+constexpr uint32_t kPairCountRow = kMaxCountedPasses * kMaxRadix;  // after the item sort's counted rows
+constexpr uint32_t kMaxCountedPairPasses = 3;
+static_assert(kPairCountRow + kMaxCountedPairPasses * kMaxRadix <= kMaxPasses * kMaxRadix, "pair count rows");
 
 }  // namespace tpd
