@@ -159,6 +159,7 @@ def time_multiview(eng, g, steps, warmup, tune=True):
     eng.setProfiling(False)
     stages = {k: round(med([s[k] for s in st]), 4) for k in st[0]}
     sorts = {k: (round(med([s[k] for s in so]), 4) if k.endswith("_ms") else so[0][k]) for k in so[0]}
+    sorts["items_kept"] = eng.batchItemCounts()[1]  # the visible items: the item sort drops the rest
     return {"views": len(views), "batched_views_per_s": round(len(views) / t_batch, 2),
             "plain_views_per_s": round(len(views) / t_plain, 2),
             "batched_over_plain": round(t_plain / t_batch, 3),
@@ -488,15 +489,18 @@ def main():
     log(f"[bench] sort-load: P={P2}, stages {st2}")
 
     # Roofline of the radix passes (k_onesweep) of the batched frame: the items are sorted by depth
-    # (64-bit keys: the depth key and the carried emission record; 32-bit values: 24 algorithmic
-    # bytes per element per pass, read and written), then the pairs by tile (32-bit keys and
-    # values: 16 bytes); achieved = all passes' bytes / all passes' time (the time-weighted mean over
-    # the frame's launches).
+    # (64-bit keys: the depth key and the carried emission record; 32-bit values: 12 bytes per
+    # element read and 12 written per pass; the first pass reads every item and writes the visible
+    # ones, the later passes read and write the visible ones), then the pairs by tile (32-bit keys
+    # and values: 16 bytes); achieved = all passes' bytes / all passes' time (the time-weighted mean
+    # over the frame's launches).
     es, so = mv["eager_stage_ms"], mv["sorts"]
     tuned = mv.get("autotune") or {}
     item_shape = tuned.get("item_shape", args.sort_shape)
     pair_shape = tuned.get("pair_shape", args.sort_shape)
-    item_bytes = 24.0 * so["items"] * so["item_passes"]
+    kept = so.get("items_kept", so["items"])
+    item_bytes = (12.0 * so["items"] + 12.0 * kept + 24.0 * kept * (so["item_passes"] - 1)
+                  if so["item_passes"] else 0.0)
     pair_bytes = 16.0 * so["pairs"] * so["pair_passes"]
     pass_bytes = item_bytes + pair_bytes
     t_passes = (so["item_passes_ms"] + so["pair_passes_ms"]) / 1000.0
@@ -509,7 +513,7 @@ def main():
                 "traffic_source": traffic["source"] if traffic else None,
                 "kernel": "k_onesweep (item sort: 64-bit keys; pair sort: 32-bit keys)",
                 "sort_shape": {"items": shape_desc(item_shape), "pairs": shape_desc(pair_shape)},
-                "items": so["items"], "item_passes": so["item_passes"],
+                "items": so["items"], "items_kept": kept, "item_passes": so["item_passes"],
                 "pairs": so["pairs"], "pair_passes": so["pair_passes"],
                 "item_passes_gbs": round(item_bytes / (so["item_passes_ms"] / 1000.0) / 1e9, 1)
                 if so["item_passes_ms"] > 0 else 0.0,

@@ -234,6 +234,11 @@ int tpd_batch_backward(tpd_ctx* ctx, const float* d_dL_dimages, const tpd_scene_
 int tpd_batch_step(tpd_ctx* ctx, const float* d_dL_dimages, const tpd_scene_grads* grads, int accumulate, void* stream);
 /* Pair count of the last batch frame (all views). */
 int tpd_batch_get_pair_count(tpd_ctx* ctx, uint32_t* P);
+/* Items of the last batch frame (every (Gaussian, view) of its scenes) and how many of them its
+ * item sort kept: the visible ones, whose tile count is non-zero (the sort drops the others in its
+ * first pass, and the later passes and the pair emission run on the kept ones).  No counterpart in
+ * the reference, which renders one camera per frame. */
+int tpd_batch_get_item_counts(tpd_ctx* ctx, uint32_t* items, uint32_t* kept);
 /* Where view `view` lies in the batch's index spaces: its first global tile (keys carry global
  * tiles), its first item (pair values are items; item - item_base = Gaussian index) and its first
  * pixel in the concatenated images. */

@@ -61,7 +61,7 @@ def check_batch(beng, plain, records, degs, views, name, orc=None):
     assert keys.shape[0] == P
     tiles = (keys >> np.uint64(32)).astype(np.int64)
     assert np.all(np.diff(tiles) >= 0), "global tiles not sorted"
-    covered = 0
+    covered = visible = 0
     for vi, view in enumerate(views):
         sid, _, _, _, w, h = view
         tb, ib, pb = beng.batchViewLayout(vi)
@@ -72,6 +72,7 @@ def check_batch(beng, plain, records, degs, views, name, orc=None):
         local_v = vv - np.uint32(ib)
         plain.uploadRecords(records[sid], degs[sid])
         pk, pv, pimg, prgba = plain_frame(plain, view)
+        visible += int(np.count_nonzero(plain.readTiles()[0]))
         assert np.array_equal(local_k, pk), (name, vi, "keys")
         assert np.array_equal(local_v, pv), (name, vi, "values")
         img = beng.readBatchImage(vi)
@@ -84,6 +85,10 @@ def check_batch(beng, plain, records, degs, views, name, orc=None):
             assert np.array_equal(local_k, ref["keys"]) and np.array_equal(local_v, ref["vals"]), (name, "oracle")
             assert np.abs(img - ref["image"]).max() <= 1e-5, (name, "oracle image")
     assert covered == P, (name, covered, P)
+    # the item sort keeps exactly the visible items: those with tiles in the plain frames
+    items, kept = beng.batchItemCounts()
+    assert items == sum(records[v[0]].shape[0] for v in views), (name, "items")
+    assert kept == visible, (name, "kept items", kept, visible)
 
 
 def test_batch_views_of_one_scene(engines, orc):

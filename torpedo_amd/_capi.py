@@ -88,6 +88,7 @@ SIGNATURES = {
     "tpd_batch_backward": (_i32, [_vp, _vp, ctypes.POINTER(TpdSceneGrads), _i32, _vp]),
     "tpd_batch_step": (_i32, [_vp, _vp, ctypes.POINTER(TpdSceneGrads), _i32, _vp]),
     "tpd_batch_get_pair_count": (_i32, [_vp, ctypes.POINTER(_u32)]),
+    "tpd_batch_get_item_counts": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "tpd_batch_view_layout": (_i32, [_vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32),
                                      ctypes.POINTER(ctypes.c_uint64)]),
     "tpd_batch_read_image_f32": (_i32, [_vp, _u32, _vp]),

@@ -104,6 +104,8 @@ struct BatchState {
     bool rendered = false;      // last frame validated
     bool kept_bwd_state = false;
     uint32_t last_pairs = 0;
+    bool items_dropped = false;  // the recorded frame's item sort drops the culled items
+    uint32_t last_kept = 0;      // the last frame's items kept by its item sort (the visible ones)
     hipEvent_t ev[11] = {};  // eager stage events [0..6], frame bracket [7], [8], after each sort's histogram [9], [10]
     bool timed = false;
     uint32_t item_passes = 0, pair_passes = 0;  // radix passes of the recorded frame's two sorts
@@ -448,9 +450,15 @@ static void record_forward(BatchState* b, bool keep_bwd, hipStream_t s, bool eve
         // emission record) rides along.  The item sort has a block shape of its own.
         const uint32_t pair_shape = b->ws.shape;
         b->ws.shape = b->item_shape;
+        // with the counts from the item kernel, which counts the visible items only, the first
+        // pass drops the culled items (zero high word) and the rest of the frame runs on the
+        // visible ones (their number in kCtrItemsKept)
+        const bool counted = a.hist_counts != nullptr;
+        b->items_dropped = counted;
         items_alt = launch_sort_pairs(b->ws, b->d_ikeys, b->d_ivals, b->d_ikeys_alt, b->d_ivals_alt, nullptr,
                                       b->nitems, b->lo_bits(), s, events ? b->ev[9] : nullptr, 32u, nullptr, 0u,
-                                      a.hist_counts != nullptr);
+                                      counted, counted ? &b->d_counters[kCtrItemsKept] : nullptr);
+        a.kept_items = counted ? &b->d_counters[kCtrItemsKept] : nullptr;
         b->ws.shape = pair_shape;
     } else if (events)
         hipEventRecord(b->ev[9], s);
@@ -588,6 +596,7 @@ static int validate(tpd_ctx* c, BatchState* b) {
         if (P > kMaxSortKeys) return fail(TPD_ERR_CAPACITY, "tpd_batch: more than 2^30 - 1 tile/Gaussian pairs");
         if (!b->h_counters[kCtrOverflow]) {
             b->last_pairs = P;
+            b->last_kept = b->items_dropped ? b->h_counters[kCtrItemsKept] : b->nitems;
             b->rendered = true;
             b->kept_bwd_state = b->pending_mode != 0;
             return TPD_OK;
@@ -979,6 +988,17 @@ int tpd_batch_get_sort_times(tpd_ctx* c, float* ms, uint32_t* counts) {
         counts[3] = b->pair_passes;
     }
     return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_batch_get_sort_times");
+}
+
+int tpd_batch_get_item_counts(tpd_ctx* c, uint32_t* items, uint32_t* kept) {
+    if (!c || !items || !kept) return fail(TPD_ERR_INVALID, "tpd_batch_get_item_counts: bad argument");
+    hipSetDevice(c->device);
+    BatchState* b = state(c);
+    int rc = validate(c, b);
+    if (rc != TPD_OK) return rc;
+    *items = b->nitems;
+    *kept = b->last_kept;
+    return TPD_OK;
 }
 
 // Times every radix-pass block shape on the batch's own items and pairs (both sorts of full

@@ -136,7 +136,7 @@ __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const Batc
         a.item_keys[item] = key;
         a.item_vals[item] = item;
         if (a.tiles) a.tiles[item] = p.tiles;
-        if (L) {
+        if (L && vis) {  // the item sort drops the culled items in its first pass
             const uint32_t k32 = (uint32_t)key;  // the depth key: the bits the item sort orders by
             const uint32_t dmask = (1u << a.hist_bits) - 1u;
             for (uint32_t q = 0; q < a.hist_passes; ++q)
@@ -274,16 +274,30 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     for (uint32_t x = blockIdx.x * kPreBlock + tid; x < a.pair_desc0_words; x += gridDim.x * kPreBlock)
         a.hist_desc0[x] = 0u;
     if (a.pair_desc0_words && blockIdx.x == 0 && tid < kMaxPasses) a.hist_tickets[tid] = 0u;
+    // The sorted items are the visible ones (the item sort dropped the culled).  The grid is sized
+    // for all items; the blocks past the visible items' partitions leave before taking a ticket
+    // (the tickets, one atomic each on one address, pace the blocks' starts), so that exactly the
+    // first nparts blocks take tickets 0 .. nparts - 1.
+    const uint32_t nitems =
+        __builtin_amdgcn_readfirstlane(a.kept_items ? min(*a.kept_items, a.nitems) : a.nitems);
+    const uint32_t nparts = (nitems + kEmitItems - 1) / kEmitItems;
+    if (blockIdx.x >= nparts) {
+        if (blockIdx.x == 0 && tid == 0) {  // no visible item: the totals the last partition would store
+            a.counters[kCtrPairs] = 0u;
+            a.counters[kCtrOverflow] = 0u;
+            a.counters[kCtrSortPairs] = 0u;
+        }
+        return;
+    }
     if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
     __syncthreads();
     const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);
-    const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
-    if (part >= nparts) return;  // the grid has nparts blocks: never true in a consistent state
+    if (part >= nparts) return;  // never: the first nparts blocks share nparts tickets
     const uint32_t i0 = part * kEmitItems + tid * kEmitPerThread;
 
     uint32_t item[kEmitPerThread];
     uint64_t rect[kEmitPerThread];
-    if (i0 + kEmitPerThread <= a.nitems) {
+    if (i0 + kEmitPerThread <= nitems) {
 #pragma unroll
         for (uint32_t j = 0; j < kEmitPerThread; j += 4) {  // i0 is a multiple of 4
             const uint4 v = *reinterpret_cast<const uint4*>(a.sorted_items + i0 + j);
@@ -295,8 +309,8 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < kEmitPerThread; ++j) {
-            item[j] = i0 + j < a.nitems ? a.sorted_items[i0 + j] : 0u;
-            rect[j] = i0 + j < a.nitems ? a.sorted_keys64[i0 + j] : 0ull;
+            item[j] = i0 + j < nitems ? a.sorted_items[i0 + j] : 0u;
+            rect[j] = i0 + j < nitems ? a.sorted_keys64[i0 + j] : 0ull;
         }
     }
 #pragma unroll

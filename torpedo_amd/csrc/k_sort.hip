@@ -126,9 +126,11 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys, 
 }
 
 // One block per pass: exclusive scan of the 2^BITS digit counts summed over their kCountCopies
-// copies (each thread takes a contiguous run of digits); every copy is then re-zeroed.
+// copies (each thread takes a contiguous run of digits); every copy is then re-zeroed.  With
+// total_out, pass 0's block stores the number of keys counted there.
 template <uint32_t BITS>
-__global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets) {
+__global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
+                                                      uint32_t* __restrict__ total_out) {
     constexpr uint32_t R = 1u << BITS;
     constexpr uint32_t DPT = R / 256;
     __shared__ uint32_t s_w[4];
@@ -156,6 +158,7 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
         offsets[p * R + t * DPT + i] = off;
         off += c[i];
     }
+    if (total_out && p == 0 && t == 255) *total_out = off;
 }
 
 // ------------------------------------------------------------------ one pass
@@ -163,7 +166,9 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
 // Key widths of a pass: kWide reads and writes 64-bit keys; kNarrow reads 64-bit keys and writes
 // the 32-bit key (compacted key >> drop), possible once the passes before it have consumed the bits
 // below `drop` and the rest fit in 32 bits; kNarrowed reads and writes such 32-bit keys.
-enum : uint32_t { kWide = 0, kNarrow = 1, kNarrowed = 2 };
+// kWideDrop is kWide that also drops the keys whose high word is zero (the batch's culled items,
+// which the digit counts leave out): they are ranked after every digit and never written.
+enum : uint32_t { kWide = 0, kNarrow = 1, kNarrowed = 2, kWideDrop = 3 };
 
 struct OnesweepArgs {
     const void* keys_in;
@@ -193,6 +198,8 @@ struct OnesweepLds {
     uint32_t bstart[R];
     uint32_t adj[R];
     uint32_t wtot[8];
+    uint32_t wcull[8];          // kWideDrop: per wave dropped keys (then wave offsets among them)
+    uint32_t nkeep;             // kWideDrop: the partition's kept keys (the dropped ones follow)
     uint32_t part;
     KT keys[kTileKeys];         // the partition reordered by digit (then its values)
 };
@@ -237,7 +244,9 @@ __device__ __forceinline__ uint32_t onesweep_lookback(const uint32_t* desc, uint
 // bookkeeping and the look-back.
 template <uint32_t BITS, uint32_t BLOCK, uint32_t ITEMS, uint32_t MODE>
 __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
-    using KT = typename std::conditional<MODE == kWide, uint64_t, uint32_t>::type;  // key type after the load
+    constexpr bool DROP = MODE == kWideDrop;
+    constexpr bool WIDE = MODE == kWide || DROP;
+    using KT = typename std::conditional<WIDE, uint64_t, uint32_t>::type;  // key type after the load
     using Lds = OnesweepLds<BITS, BLOCK, ITEMS, KT>;
     constexpr uint32_t R = Lds::R;
     constexpr uint32_t kWaves = Lds::kWaves;
@@ -253,6 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) L.part = atomicAdd(a.ticket, 1u);
     for (uint32_t x = tid; x < kWaves * R / 2; x += BLOCK) reinterpret_cast<uint32_t*>(&L.whist[0][0])[x] = 0;
+    if (DROP && tid < 8) L.wcull[tid] = 0;
     __syncthreads();
     const uint32_t part = __builtin_amdgcn_readfirstlane(L.part);  // block-uniform: keep it scalar
     if (part >= nparts) return;
@@ -260,13 +270,13 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
         for (uint32_t d = tid; d < R; d += BLOCK) a.desc_next[part * R + d] = 0;
 
     // Load ITEMS keys and values per thread, wave-striped: element e = wave*(ITEMS*64) + j*64 +
-    // lane; past the end keys are all ones.  Every partition but the last is full: its loads and
-    // stores need no bounds checks.
+    // lane; past the end keys are all ones (kWideDrop: zero, i.e. dropped).  Every partition but the
+    // last is full: its loads (and, without dropped keys, its stores) need no bounds checks.
     const uint32_t base = part * kTileKeys;
     const uint32_t wbase = base + wave * (ITEMS * kWave) + lane;
     const bool full = base + kTileKeys <= n;
     const auto load_key = [&](uint32_t idx) -> KT {
-        if constexpr (MODE == kWide) return static_cast<const uint64_t*>(a.keys_in)[idx];
+        if constexpr (WIDE) return static_cast<const uint64_t*>(a.keys_in)[idx];
         if constexpr (MODE == kNarrow)
             return (uint32_t)(compact_key(static_cast<const uint64_t*>(a.keys_in)[idx], a.lo_bits, a.lo_base) >> a.drop);
         return static_cast<const uint32_t*>(a.keys_in)[idx];
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
             const uint32_t idx = wbase + j * kWave;
-            k[j] = idx < n ? load_key(idx) : ~KT(0);
+            k[j] = idx < n ? load_key(idx) : (DROP ? KT(0) : ~KT(0));
         }
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
@@ -295,10 +305,11 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     // each group advances the wave's running digit counter.  Digits are recomputed from the keys
     // wherever they are needed (a few instructions) rather than kept in registers or LDS.
     const auto digit = [&](KT key) -> uint32_t {
-        if constexpr (MODE == kWide) return digit_of<BITS>(compact_key(key, a.lo_bits, a.lo_base), a.shift) & a.digit_mask;
+        if constexpr (WIDE) return digit_of<BITS>(compact_key(key, a.lo_bits, a.lo_base), a.shift) & a.digit_mask;
         else return (uint32_t)(key >> (a.shift - a.drop)) & a.digit_mask;
     };
     const uint64_t lt = lanemask_lt();
+    const auto dropped = [&](KT key) { return DROP && (uint32_t)((uint64_t)key >> 32) == 0u; };
     uint32_t rank[ITEMS];
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
@@ -310,12 +321,34 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
             const uint64_t bal = ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const uint32_t below = (uint32_t)__popcll(peers & lt);
-        const uint32_t pre = L.whist[wave][d];
-        if (below == 0) L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
-        rank[j] = pre + below;
+        if constexpr (DROP) {
+            // the dropped keys of the wave form one more group, after every digit
+            const bool cull = dropped(k[j]);
+            const uint64_t cb = ballot(cull);
+            peers = cull ? cb : (peers & ~cb);
+            const uint32_t below = (uint32_t)__popcll(peers & lt);
+            const uint32_t pre = cull ? L.wcull[wave] : (uint32_t)L.whist[wave][d];
+            if (below == 0) {
+                if (cull) L.wcull[wave] = pre + (uint32_t)__popcll(peers);
+                else L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
+            }
+            rank[j] = pre + below;
+        } else {
+            const uint32_t below = (uint32_t)__popcll(peers & lt);
+            const uint32_t pre = L.whist[wave][d];
+            if (below == 0) L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
+            rank[j] = pre + below;
+        }
     }
     __syncthreads();
+    if (DROP && tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t c = L.wcull[w];
+            L.wcull[w] = acc;
+            acc += c;
+        }
+    }
 
     // Per digit: wave exclusive offsets and the partition's digit counts (published); then the
     // digits' block-local starts by an exclusive scan over the digit-owning threads.
@@ -349,6 +382,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
             L.bstart[tid * DPT + i] = tstart;
             tstart += tot[i];
         }
+        if (DROP && tid == DTH - 1) L.nkeep = tstart;  // the kept keys of all digits
     }
     __syncthreads();
 
@@ -356,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = digit(k[j]);
-        const uint32_t lp = L.bstart[d] + L.whist[wave][d] + rank[j];
+        const uint32_t lp = dropped(k[j]) ? L.nkeep + L.wcull[wave] + rank[j] : L.bstart[d] + L.whist[wave][d] + rank[j];
         rank[j] = lp;
         L.keys[lp] = k[j];
     }
@@ -378,14 +412,16 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 
     // Keys out, each to its digit's global start + its place in the partition; the global position
     // of each output slot is kept for the values (the keys' LDS is reused for them).
-    const uint32_t nvalid = min(kTileKeys, n - base);
+    // (the kept keys come first in LDS: the past-the-end keys are in the last digit, or dropped)
+    const uint32_t nvalid = DROP ? L.nkeep : min(kTileKeys, n - base);
+    const bool all_valid = full && !DROP;
     uint32_t gpos[ITEMS];
 #pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
         const KT key = L.keys[i];
         gpos[t] = L.adj[digit(key)] + i;
-        if (full || i < nvalid) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
+        if (all_valid || i < nvalid) static_cast<KT*>(a.keys_out)[gpos[t]] = key;
     }
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
@@ -395,18 +431,23 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
         const uint32_t i = t * BLOCK + tid;
-        if (full || i < nvalid) a.vals_out[gpos[t]] = s_vals[i];
+        if (all_valid || i < nvalid) a.vals_out[gpos[t]] = s_vals[i];
     }
 }
 
 // Shapes the tuner chooses from: {digit bits, threads, keys per thread}.
-// kernel[mode] for the key widths kWide, kNarrow, kNarrowed.
+// kernel[mode] for the key widths kWide, kNarrow, kNarrowed and kWideDrop.
 struct SortShape {
     uint32_t bits, block, items;
-    void (*kernel[3])(const OnesweepArgs);
+    void (*kernel[4])(const OnesweepArgs);
 };
-#define TPD_SHAPE(B, T, I) \
-    { B, T, I, { k_onesweep<B, T, I, kWide>, k_onesweep<B, T, I, kNarrow>, k_onesweep<B, T, I, kNarrowed> } }
+#define TPD_SHAPE(B, T, I)                                                                              \
+    {                                                                                                   \
+        B, T, I, {                                                                                      \
+            k_onesweep<B, T, I, kWide>, k_onesweep<B, T, I, kNarrow>, k_onesweep<B, T, I, kNarrowed>, \
+                k_onesweep<B, T, I, kWideDrop>                                                          \
+        }                                                                                               \
+    }
 static const SortShape kShapes[] = {
     TPD_SHAPE(8, 256, 16), TPD_SHAPE(8, 256, 8),  TPD_SHAPE(8, 256, 12), TPD_SHAPE(8, 512, 8),
     TPD_SHAPE(8, 512, 12), TPD_SHAPE(8, 256, 20), TPD_SHAPE(9, 512, 12), TPD_SHAPE(9, 512, 8),
@@ -430,7 +471,7 @@ static void launch_hist(uint32_t hb, uint32_t passes, hipStream_t s, const KT* k
     const KT key_mask = key_bits >= 8 * sizeof(KT) ? ~KT(0) : (KT(1) << key_bits) - KT(1);
     hipLaunchKernelGGL((k_sort_hist<BITS, KT>), dim3(hb), dim3(256), 0, s, keys, count_ptr, n_max, passes, lo_bits,
                        lo_base, key_mask, tile, ws.partials, ws.desc, ws.tickets);
-    hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+    hipLaunchKernelGGL(k_sort_offsets<BITS>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets, nullptr);
 }
 
 // The digit bits of the pass at `shift` that lie below key_bits.
@@ -446,7 +487,8 @@ static uint32_t hist_blocks(uint32_t n_max) {
 
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
-                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop, uint32_t lo_base, bool counted) {
+                       hipEvent_t after_hist, uint32_t lo_bits, uint32_t* narrow_drop, uint32_t lo_base, bool counted,
+                       uint32_t* kept) {
     const SortShape& sh = kShapes[ws.shape % kNumShapes];
     const uint32_t R = 1u << sh.bits;
     const uint32_t tile = sh.block * sh.items;
@@ -455,9 +497,10 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     if (parts == 0 || passes == 0 || passes > kMaxPasses) return false;
 
     const uint32_t hb = hist_blocks(n_max);
+    if (!counted) kept = nullptr;  // the histogram pass counts every key
     if (counted) {
-        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
-        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets);
+        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets, kept);
+        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, ws.partials, ws.offsets, kept);
     } else if (sh.bits == 8) {
         launch_hist<8>(hb, passes, s, keys, count_ptr, n_max, lo_bits, lo_base, key_bits, tile, ws);
     } else {
@@ -475,6 +518,7 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
             }
         *narrow_drop = narrow_pass < passes ? narrow_pass * sh.bits : kKeysWide;
     }
+    if (narrow_pass == 0) kept = nullptr;  // the dropping pass reads and writes 64-bit keys
     const uint64_t* kin = keys;
     const uint32_t* vin = vals;
     uint64_t* kout = keys_alt;
@@ -482,13 +526,14 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
     for (uint32_t p = 0; p < passes; ++p) {
         OnesweepArgs o;
         o.keys_in = kin; o.vals_in = vin; o.keys_out = kout; o.vals_out = vout;
-        o.count_ptr = count_ptr; o.n_imm = n_max;
+        o.count_ptr = (kept && p > 0) ? kept : count_ptr;
+        o.n_imm = n_max;
         o.shift = p * sh.bits;
         o.lo_bits = lo_bits;
         o.lo_base = lo_base;
         o.drop = narrow_pass < passes ? narrow_pass * sh.bits : 0u;
         o.digit_mask = pass_digit_mask(sh.bits, key_bits, o.shift);
-        const uint32_t mode = p < narrow_pass ? kWide : (p == narrow_pass ? kNarrow : kNarrowed);
+        const uint32_t mode = p < narrow_pass ? ((kept && p == 0) ? kWideDrop : kWide) : (p == narrow_pass ? kNarrow : kNarrowed);
         o.offsets = ws.offsets + p * R;
         o.desc = ws.desc + (size_t)(p & 1u) * ws.parts_cap * R;
         o.desc_next = ws.desc + (size_t)((p + 1) & 1u) * ws.parts_cap * R;
@@ -512,8 +557,8 @@ bool launch_sort_pairs32(SortWorkspace& ws, uint32_t* keys, uint32_t* vals, uint
     const uint32_t hb = hist_blocks(n_max);
     if (counted) {
         uint32_t* counts = ws.partials + kPairCountRow;
-        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, counts, ws.offsets);
-        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, counts, ws.offsets);
+        if (sh.bits == 8) hipLaunchKernelGGL(k_sort_offsets<8>, dim3(passes), dim3(256), 0, s, counts, ws.offsets, nullptr);
+        else hipLaunchKernelGGL(k_sort_offsets<9>, dim3(passes), dim3(256), 0, s, counts, ws.offsets, nullptr);
     } else if (sh.bits == 8) {
         launch_hist<8>(hb, passes, s, (const uint32_t*)keys, count_ptr, n_max, 32u, 0u, key_bits, tile, ws);
     } else {

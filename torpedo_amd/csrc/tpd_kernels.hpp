@@ -14,7 +14,11 @@ enum : uint32_t {
     kCtrScanTicket2 = 4,  // partition ticket of a second scan in the frame (the batch's item offsets)
     kCtrSortPairs = 5,    // the batch's pair count for its pair sort and ranges: P, or 0 when P exceeded
                           // the capacity (that frame is rendered again)
-    kCtrCount = 16,
+    kCtrItemsKept = 32,   // the batch's visible items: those the item sort keeps (its later passes and
+                          // the emission run on them).  128 bytes past the tickets: every emission
+                          // block reads it right after its ticket atomic, and on the tickets' L2
+                          // line that read would queue behind the other blocks' atomics
+    kCtrCount = 64,
 };
 
 // The sort's depth window.  Culling keeps view depths inside [near, far], and depths in
@@ -195,7 +199,8 @@ struct BatchArgs {
     // descriptors (pair_desc0_words, for the pair capacity) and the tickets
     uint32_t* pair_counts;
     uint32_t pair_bits, pair_passes, pair_desc0_words;
-    const uint64_t* sorted_keys64;  // emission input: the item keys in depth order ...
+    const uint32_t* kept_items;     // emission input: the number of sorted items (the visible ones) ...
+    const uint64_t* sorted_keys64;  // ... the item keys in depth order ...
     const uint32_t* sorted_items;   // ... and their items
     uint32_t* keys;               // pairs: global tile
     uint32_t* vals;               //        item
@@ -296,9 +301,12 @@ uint32_t sort_shape_passes(uint32_t shape, uint32_t key_bits);  // passes for ke
 bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                        const uint32_t* count_ptr, uint32_t n_max, uint32_t key_bits, hipStream_t s,
                        hipEvent_t after_hist = nullptr, uint32_t lo_bits = 32, uint32_t* narrow_drop = nullptr,
-                       uint32_t lo_base = 0, bool counted = false);
+                       uint32_t lo_base = 0, bool counted = false, uint32_t* kept = nullptr);
 // With counted, the digit counts are already in ws.partials and the first pass's descriptors and the
 // tickets already cleared (by the kernel that made the keys): the histogram kernel is not launched.
+// With kept as well (and 64-bit keys in the first pass), the counts leave out the keys whose high
+// word is zero and the first pass drops those keys: *kept receives the number of keys kept, the
+// later passes sort those, and the result holds the kept keys in order (stable), nothing after.
 constexpr uint32_t kMaxCountedPasses = 4;  // passes whose counts the item kernel can make
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
 // moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.  With counted,

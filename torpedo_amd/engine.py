@@ -592,6 +592,13 @@ class GaussianEngine:
         check(lib().tpd_batch_get_pair_count(self._ctx, ctypes.byref(v)))
         return int(v.value)
 
+    def batchItemCounts(self) -> tuple:
+        """(items, kept) of the last batch frame: all (Gaussian, view) items, and the visible ones
+        that its item sort kept (tpd_batch_get_item_counts)."""
+        items, kept = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().tpd_batch_get_item_counts(self._ctx, ctypes.byref(items), ctypes.byref(kept)))
+        return int(items.value), int(kept.value)
+
     def batchViewLayout(self, view: int):
         tb, ib, pb = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
         check(lib().tpd_batch_view_layout(self._ctx, int(view), ctypes.byref(tb), ctypes.byref(ib), ctypes.byref(pb)))
