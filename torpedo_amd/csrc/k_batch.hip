@@ -42,11 +42,12 @@ void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, u
     hipLaunchKernelGGL(k_batch_mats, dim3((ne + 63) / 64, nviews), dim3(64), 0, s, scenes, views, mats, counters);
 }
 
-__device__ __forceinline__ bool project_in_view(const float* rec, const EntityMats& m, const BatchViewDev& vw,
-                                                uint32_t sh_degree, Projected& p) {
-    return project_gaussian(rec, m.vm, m.pvm, vw.cam.view, (0.5f * (float)vw.cam.width) * vw.cam.focal_ndc[0],
-                            (0.5f * (float)vw.cam.height) * vw.cam.focal_ndc[1], vw.cam.width, vw.cam.height, vw.gx,
-                            vw.gy, sh_degree, p);
+__device__ __forceinline__ bool project_in_view(const float* rec, const Cov3& cov, const EntityMats& m,
+                                                const BatchViewDev& vw, uint32_t sh_degree, Projected& p) {
+    return project_gaussian_cov(rec, [&cov] { return cov; }, m.vm, m.pvm, vw.cam.view,
+                                (0.5f * (float)vw.cam.width) * vw.cam.focal_ndc[0],
+                                (0.5f * (float)vw.cam.height) * vw.cam.focal_ndc[1], vw.cam.width, vw.cam.height,
+                                vw.gx, vw.gy, sh_degree, p);
 }
 
 // ------------------------------------------------------------------ items
@@ -98,13 +99,16 @@ __device__ __forceinline__ void count_pair_digits(ItemCountLds& L, uint32_t& all
 
 // One Gaussian's items, for every view of its scene (the body of k_batch_items); with L, the
 // digits of each item key (and, with a.pair_counts, of each visible item's pair keys) are counted
-// in the block's LDS histograms.
+// in the block's LDS histograms.  The record's 3D covariance is computed once for all views.
+// ENT: the scene has entities (a per-Gaussian entity index); without, every Gaussian of the block
+// uses the view's first entity matrices, a block-uniform address (scalar loads).
+template <bool ENT>
 __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const BatchSceneDev& sc,
                                                   const BatchViewDev* __restrict__ views,
                                                   const uint32_t* __restrict__ scene_views,
                                                   const EntityMats* __restrict__ mats, uint32_t g,
                                                   ItemCountLds* L, uint32_t& all0) {
-    const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
+    const uint32_t e = ENT ? sc.ent_idx[g] : 0u;
     const uint32_t culled_key = a.lo_bits >= 32 ? 0xFFFFFFFFu : (1u << a.lo_bits) - 1u;
 
     // The record is read once and kept in registers for every view.
@@ -117,11 +121,12 @@ __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const Batc
     }
     bool depth_out = false;  // a visible depth outside the sort's depth window
     const OpacityTerms ot = opacity_terms(rec[3]);  // the same for every view
+    const Cov3 cov = covariance3d(rec);
     for (uint32_t k = 0; k < sc.nviews; ++k) {
         const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
         const uint32_t item = vw.item_base + g;
         Projected p;
-        const bool vis = project_in_view(rec, mats[vw.mat_base + e], vw, sc.sh_degree, p);
+        const bool vis = project_in_view(rec, cov, mats[vw.mat_base + (ENT ? e : 0u)], vw, sc.sh_degree, p);
         uint64_t key = culled_key;
         if (vis) {
             store_splat(a.splats + item, true, p, ot);
@@ -184,7 +189,12 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_items(const BatchArgs a, co
     const BatchSceneDev& sc = scenes[block_scene[b]];
     const uint32_t g = (b - sc.first_block) * kPreBlock + tid;
     uint32_t all0 = 0;
-    if (g < sc.n) items_of_gaussian(a, sc, views, scene_views, mats, g, counting ? &L : nullptr, all0);
+    if (g < sc.n) {
+        if (sc.ent_idx)
+            items_of_gaussian<true>(a, sc, views, scene_views, mats, g, counting ? &L : nullptr, all0);
+        else
+            items_of_gaussian<false>(a, sc, views, scene_views, mats, g, counting ? &L : nullptr, all0);
+    }
     if (!counting) return;
     if (pairs) {
         const uint32_t s_all = wave_incl_scan(all0);

@@ -155,12 +155,43 @@ struct Projected {
     int32_t x0, y0, x1, y1;
 };
 
+// volume.slang:20-41 computeCovariance: the 3D covariance M M^T, M = R(q) diag(scale * modifier),
+// of record g.  Its products commute exactly, so (M M^T)[i][j] and [j][i] are the same float and
+// six values hold it.
+struct Cov3 {
+    float c00, c01, c02, c11, c12, c22;
+};
+__device__ __forceinline__ Cov3 covariance3d(const float* __restrict__ g) {
+    const float sx = g[8] * g[11], sy = g[9] * g[11], sz = g[10] * g[11];
+    const float qx = g[4], qy = g[5], qz = g[6], qw = g[7];
+    const float R00 = 1.0f - 2.0f * (qy * qy + qz * qz), R01 = 2.0f * (qx * qy - qw * qz),
+                R02 = 2.0f * (qx * qz + qw * qy);
+    const float R10 = 2.0f * (qx * qy + qw * qz), R11 = 1.0f - 2.0f * (qx * qx + qz * qz),
+                R12 = 2.0f * (qy * qz - qw * qx);
+    const float R20 = 2.0f * (qx * qz - qw * qy), R21 = 2.0f * (qy * qz + qw * qx),
+                R22 = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float M00 = R00 * sx, M01 = R01 * sy, M02 = R02 * sz;
+    const float M10 = R10 * sx, M11 = R11 * sy, M12 = R12 * sz;
+    const float M20 = R20 * sx, M21 = R21 * sy, M22 = R22 * sz;
+    Cov3 c;
+    c.c00 = (M00 * M00 + M01 * M01) + M02 * M02;
+    c.c01 = (M00 * M10 + M01 * M11) + M02 * M12;
+    c.c02 = (M00 * M20 + M01 * M21) + M02 * M22;
+    c.c11 = (M10 * M10 + M11 * M11) + M12 * M12;
+    c.c12 = (M10 * M20 + M11 * M21) + M12 * M22;
+    c.c22 = (M20 * M20 + M21 * M21) + M22 * M22;
+    return c;
+}
+
 // project.slang:33-90 for the Gaussian record g (60 floats: position, opacity, quaternion xyzw,
 // scale xyz + modifier, sh[48]).  Returns false for culled / degenerate / tile-less Gaussians,
-// in which case p.tiles = 0 and p.radius = 0.
-__device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, const float* vm, const float* pvm,
-                                                 const float* V, float focal_x, float focal_y, uint32_t W, uint32_t H,
-                                                 uint32_t gx, uint32_t gy, uint32_t sh_degree, Projected& p) {
+// in which case p.tiles = 0 and p.radius = 0.  cov_of() gives the record's 3D covariance (computed
+// here, past the frustum test, or once for all the views of a batch).
+template <class CovOf>
+__device__ __forceinline__ bool project_gaussian_cov(const float* __restrict__ g, CovOf cov_of, const float* vm,
+                                                     const float* pvm, const float* V, float focal_x, float focal_y,
+                                                     uint32_t W, uint32_t H, uint32_t gx, uint32_t gy,
+                                                     uint32_t sh_degree, Projected& p) {
     p.tiles = 0;
     p.radius = 0.0f;
     const float mx = g[0], my = g[1], mz = g[2];
@@ -176,27 +207,10 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
     const float inv_w = 1.0f / cw;
     const float ndc_x = cx * inv_w, ndc_y = cy * inv_w;
 
-    // volume.slang:20-41 computeCovariance
-    const float sx = g[8] * g[11], sy = g[9] * g[11], sz = g[10] * g[11];
-    const float qx = g[4], qy = g[5], qz = g[6], qw = g[7];
-    const float R00 = 1.0f - 2.0f * (qy * qy + qz * qz), R01 = 2.0f * (qx * qy - qw * qz),
-                R02 = 2.0f * (qx * qz + qw * qy);
-    const float R10 = 2.0f * (qx * qy + qw * qz), R11 = 1.0f - 2.0f * (qx * qx + qz * qz),
-                R12 = 2.0f * (qy * qz - qw * qx);
-    const float R20 = 2.0f * (qx * qz - qw * qy), R21 = 2.0f * (qy * qz + qw * qx),
-                R22 = 1.0f - 2.0f * (qx * qx + qy * qy);
-    const float M00 = R00 * sx, M01 = R01 * sy, M02 = R02 * sz;
-    const float M10 = R10 * sx, M11 = R11 * sy, M12 = R12 * sz;
-    const float M20 = R20 * sx, M21 = R21 * sy, M22 = R22 * sz;
-    const float C00 = (M00 * M00 + M01 * M01) + M02 * M02;
-    const float C01 = (M00 * M10 + M01 * M11) + M02 * M12;
-    const float C02 = (M00 * M20 + M01 * M21) + M02 * M22;
-    const float C10 = (M10 * M00 + M11 * M01) + M12 * M02;
-    const float C11 = (M10 * M10 + M11 * M11) + M12 * M12;
-    const float C12 = (M10 * M20 + M11 * M21) + M12 * M22;
-    const float C20 = (M20 * M00 + M21 * M01) + M22 * M02;
-    const float C21 = (M20 * M10 + M21 * M11) + M22 * M12;
-    const float C22 = (M20 * M20 + M21 * M21) + M22 * M22;
+    const Cov3 cv = cov_of();
+    const float C00 = cv.c00, C01 = cv.c01, C02 = cv.c02;
+    const float C10 = cv.c01, C11 = cv.c11, C12 = cv.c12;
+    const float C20 = cv.c02, C21 = cv.c12, C22 = cv.c22;
 
     // volume.slang:44-63 projectCovariance + computeJacobian (no clamp of x/z, y/z)
     const float fxz = focal_x / vz, fyz = focal_y / vz;
@@ -248,6 +262,12 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, co
     p.tiles = tiles;
     p.x0 = x0; p.y0 = y0; p.x1 = x1; p.y1 = y1;
     return true;
+}
+__device__ __forceinline__ bool project_gaussian(const float* __restrict__ g, const float* vm, const float* pvm,
+                                                 const float* V, float focal_x, float focal_y, uint32_t W, uint32_t H,
+                                                 uint32_t gx, uint32_t gy, uint32_t sh_degree, Projected& p) {
+    return project_gaussian_cov(g, [g] { return covariance3d(g); }, vm, pvm, V, focal_x, focal_y, W, H, gx, gy,
+                                sh_degree, p);
 }
 
 // Which 4x4-pixel sub-blocks of the 16x16 tile with top-left pixel (ox, oy) a splat may reach:
