@@ -275,6 +275,50 @@ def test_batch_without_gaussians(engines):
         assert not rgba[..., :3].any() and np.all(rgba[..., 3] == 255)
 
 
+def test_batch_all_items_culled(engines):
+    """Cameras that look away from the scene: every item is culled, the item sort keeps none and the
+    emission has no partition (its first block stores the zero totals); the frames before and after,
+    which see the scene, still equal the plain frames."""
+    from torpedo_amd import scenes
+    from torpedo_amd.engine import PerspectiveCamera
+    beng, plain = engines
+    beng.clearScenes()
+    g = scenes.cluster_scene(20_000, seed=11)
+    sid = beng.addScene(g)
+    sizes = [(160, 120), (96, 64), (33, 47)]
+    eyes = scenes.fibonacci_sphere(len(sizes), 8.0)
+    seen = [view_tuple(sid, eyes[i], w, h, up=(0.0, 0.0, 1.0)) for i, (w, h) in enumerate(sizes)]
+
+    def away(i, w, h):  # at distance 30 on the -z axis, looking further away from the origin
+        cam = PerspectiveCamera(w, h)
+        cam.lookAt([0.0, 0.0, -30.0 - i], [0.0, 0.0, -60.0], [0.0, 1.0, 0.0])
+        v, pv, fo = cam.uniforms()
+        return (sid, v, pv, fo, w, h)
+
+    hidden = [away(i, w, h) for i, (w, h) in enumerate(sizes)]
+    try:
+        for graph in (True, False):
+            beng.setBatchGraph(graph)
+            beng.setViews(seen)
+            beng.rasterBatch()
+            beng.syncBatch()
+            check_batch(beng, plain, [g], [3], seen, f"seen before, graph={graph}")
+            assert beng.batchPairCount > 0
+            beng.setViews(hidden)  # same sizes: the layout and the recorded frame are kept
+            beng.rasterBatch()
+            beng.syncBatch()
+            assert beng.batchPairCount == 0, graph
+            assert beng.batchItemCounts() == (g.shape[0] * len(sizes), 0), graph
+            for vi in range(len(sizes)):
+                assert not beng.readBatchImage(vi).any(), (graph, vi)
+            beng.setViews(seen)
+            beng.rasterBatch()
+            beng.syncBatch()
+            check_batch(beng, plain, [g], [3], seen, f"seen after, graph={graph}")
+    finally:
+        beng.setBatchGraph(True)
+
+
 def _plain_grads(plain, rec, deg, views, dls):
     """Sum over views of the plain backward for the scene rec."""
     n = rec.shape[0]
