@@ -254,13 +254,16 @@ constexpr uint32_t kEmitPerThread = kEmitItems / kPreBlock;
 static_assert(kEmitPerThread % 4 == 0, "items are loaded four at a time");
 
 // Pairs of the block's small rectangles staged per chunk before they are written out: the large
-// items' tables (start, value, rectangle) are done with by then and lend their space.
-constexpr uint32_t kEmitStage = 2 * kEmitItems;
+// items' tables (start, rectangle; their values are re-read from the sorted items) are done with
+// by then and lend their space.
+// 2 x 2,048 - 256 pairs: with the large items' tables (24 KB) in the same space, the kernel's LDS
+// stays under 40 KB, four blocks per CU.
+constexpr uint32_t kEmitStage = 2 * kEmitItems - kPreBlock;
 constexpr uint32_t kEmitMaxChunks = 4;
 constexpr uint32_t kNoPair = 0xFFFFFFFFu;  // an empty staging slot (no item has this index)
 union EmitLds {
     struct {
-        uint32_t start[kEmitItems], val[kEmitItems];
+        uint32_t start[kEmitItems];
         uint64_t rect[kEmitItems];
     } big;
     struct {
@@ -275,7 +278,6 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     __shared__ EmitLds L;
     __shared__ uint64_t s_base;
     uint32_t* const s_start = L.big.start;
-    uint32_t* const s_val = L.big.val;
     uint64_t* const s_rect = L.big.rect;
 
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
@@ -358,7 +360,6 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
             big_incl += t[j];
             s_start[o] = start;
             s_rect[o] = rect[j];
-            s_val[o] = item[j];
         }
         s_big_incl[o] = big_incl;
         start += t[j];
@@ -380,7 +381,9 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     }
     __syncthreads();
     const uint64_t base = s_base;
-    emit_large_rects(s_big_incl, s_start, s_rect, s_val, kEmitItems, big_total, base, a.cap, a.keys, a.vals);
+    // (a large item's value is read back from the sorted items, which this block has just read)
+    emit_large_rects(s_big_incl, s_start, s_rect, a.sorted_items + part * kEmitItems, kEmitItems, big_total, base,
+                     a.cap, a.keys, a.vals);
     // The small rectangles' pairs (at most kOwnTiles per item, written by the item's thread) go
     // through LDS, chunk by chunk of the block's output range, so that the stores to HBM are
     // consecutive across lanes; slots of the large items' pairs stay empty and are skipped.  A

@@ -674,10 +674,11 @@ __device__ __forceinline__ void stage_own_rect(uint32_t tiles, uint64_t rect, ui
 }
 
 // The large items of a block: s_big_incl[o] is the inclusive count of large tiles over the block's
-// items 0..o (n of them), s_start/s_rect/s_val the block-local first pair, record and value of a
-// large item o.
+// items 0..o (n of them), s_start/s_rect the block-local first pair and the record of a large item
+// o, item_vals[o] its value (the block's sorted items).
 __device__ __forceinline__ void emit_large_rects(const uint32_t* s_big_incl, const uint32_t* s_start,
-                                                 const uint64_t* s_rect, const uint32_t* s_val, uint32_t n,
+                                                 const uint64_t* s_rect, const uint32_t* __restrict__ item_vals,
+                                                 uint32_t n,
                                                  uint32_t big_total, uint64_t base, uint64_t cap,
                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     for (uint32_t j = threadIdx.x; j < big_total; j += kPreBlock) {
@@ -703,7 +704,7 @@ __device__ __forceinline__ void emit_large_rects(const uint32_t* s_big_incl, con
         const uint64_t gpos = base + s_start[owner] + k;
         if (gpos < cap) {
             keys[gpos] = emit_rect_tile0(rect) + ry * emit_rect_gx(rect) + rx;
-            vals[gpos] = s_val[owner];
+            vals[gpos] = item_vals[owner];
         }
     }
 }
