@@ -490,16 +490,16 @@ def main():
 
     # Roofline of the radix passes (k_onesweep) of the batched frame: the items are sorted by depth
     # (64-bit keys: the depth key and the carried emission record; 32-bit values: 12 bytes per
-    # element read and 12 written per pass; the first pass reads every item and writes the visible
-    # ones, the later passes read and write the visible ones), then the pairs by tile (32-bit keys
-    # and values: 16 bytes); achieved = all passes' bytes / all passes' time (the time-weighted mean
-    # over the frame's launches).
+    # element read and 12 written per pass; the first pass reads every item's key, its value being
+    # the item's index, and writes the visible items, the later passes read and write the visible
+    # ones), then the pairs by tile (32-bit keys and values: 16 bytes); achieved = all passes' bytes
+    # / all passes' time (the time-weighted mean over the frame's launches).
     es, so = mv["eager_stage_ms"], mv["sorts"]
     tuned = mv.get("autotune") or {}
     item_shape = tuned.get("item_shape", args.sort_shape)
     pair_shape = tuned.get("pair_shape", args.sort_shape)
     kept = so.get("items_kept", so["items"])
-    item_bytes = (12.0 * so["items"] + 12.0 * kept + 24.0 * kept * (so["item_passes"] - 1)
+    item_bytes = (8.0 * so["items"] + 12.0 * kept + 24.0 * kept * (so["item_passes"] - 1)
                   if so["item_passes"] else 0.0)
     pair_bytes = 16.0 * so["pairs"] * so["pair_passes"]
     pass_bytes = item_bytes + pair_bytes

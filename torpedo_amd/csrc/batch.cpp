@@ -406,6 +406,9 @@ static BatchArgs forward_args(BatchState* b, bool keep_bwd) {
             a.pair_desc0_words = pparts << pbits;
         }
     }
+    // with its counts from the item kernel the item sort drops the culled items and its first pass
+    // takes each item's index as its value: the item kernel writes no values
+    if (a.hist_counts) a.item_vals = nullptr;
     a.keys = b->d_keys;
     a.vals = b->d_vals;
     a.ranges = b->d_ranges;

@@ -139,7 +139,7 @@ __device__ __forceinline__ void items_of_gaussian(const BatchArgs& a, const Batc
             if (L && a.pair_counts) count_pair_digits(*L, all0, tile0, vw.gx, w, h, a.pair_bits, a.pair_passes);
         }
         a.item_keys[item] = key;
-        a.item_vals[item] = item;
+        if (a.item_vals) a.item_vals[item] = item;
         if (a.tiles) a.tiles[item] = p.tiles;
         if (L && vis) {  // the item sort drops the culled items in its first pass
             const uint32_t k32 = (uint32_t)key;  // the depth key: the bits the item sort orders by

@@ -167,7 +167,8 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
 // the 32-bit key (compacted key >> drop), possible once the passes before it have consumed the bits
 // below `drop` and the rest fit in 32 bits; kNarrowed reads and writes such 32-bit keys.
 // kWideDrop is kWide that also drops the keys whose high word is zero (the batch's culled items,
-// which the digit counts leave out): they are ranked after every digit and never written.
+// which the digit counts leave out): they are ranked after every digit and never written.  Its
+// values are the keys' indices (the batch's items), generated rather than read.
 enum : uint32_t { kWide = 0, kNarrow = 1, kNarrowed = 2, kWideDrop = 3 };
 
 struct OnesweepArgs {
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) k[j] = load_key(wbase + j * kWave);
 #pragma unroll
-        for (uint32_t j = 0; j < ITEMS; ++j) v[j] = a.vals_in[wbase + j * kWave];
+        for (uint32_t j = 0; j < ITEMS; ++j) v[j] = DROP ? wbase + j * kWave : a.vals_in[wbase + j * kWave];
     } else {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
         for (uint32_t j = 0; j < ITEMS; ++j) {
             const uint32_t idx = wbase + j * kWave;
-            v[j] = idx < n ? a.vals_in[idx] : 0u;
+            v[j] = idx < n ? (DROP ? idx : a.vals_in[idx]) : 0u;
         }
     }
 

@@ -183,7 +183,7 @@ struct BatchArgs {
     uint64_t* item_keys;          // per item: low word the depth key ((depth bits - lo_base) in lo_bits
                                   // bits, all ones within lo_bits when culled), high word the emission
                                   // record as inline_rect encodes it
-    uint32_t* item_vals;          // per item: the item itself
+    uint32_t* item_vals;          // per item: the item itself (null when the item sort generates them)
     uint32_t lo_bits, lo_base;
     // the item sort's digit histograms, counted by the item kernel (the sort then skips its own):
     // counts as k_sort_hist leaves them (copy blockIdx mod kCountCopies), digit width and passes of
@@ -307,6 +307,8 @@ bool launch_sort_pairs(SortWorkspace& ws, uint64_t* keys, uint32_t* vals, uint64
 // With kept as well (and 64-bit keys in the first pass), the counts leave out the keys whose high
 // word is zero and the first pass drops those keys: *kept receives the number of keys kept, the
 // later passes sort those, and the result holds the kept keys in order (stable), nothing after.
+// The first pass then also takes each key's index as its value: vals is not read, only used as
+// the later passes' buffer.
 constexpr uint32_t kMaxCountedPasses = 4;  // passes whose counts the item kernel can make
 // The same sort of 32-bit keys (key bits at and above key_bits zero) with 32-bit values: every pass
 // moves 8 bytes per pair.  Returns true when the result is in keys_alt / vals_alt.  With counted,
