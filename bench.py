@@ -41,9 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_onesweep launch of the batched frame (mean over its launches), measured with
 # FETCH_SIZE x 2 + WRITE_SIZE for the (item sort, pair sort) block shapes named; other shapes: null.
 ONESWEEP_TRAFFIC = {
-    (6, 8): {"bytes_per_launch": 917_600_000,
-             "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 857.1 MB, later "
-                       "2 x 708.2 MB; pair passes 2 x 1,157.2 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
+    (6, 8): {"bytes_per_launch": 893_200_000,
+             "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 731.9 MB, later "
+                       "2 x 708.0 MB; pair passes 2 x 1,159.1 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
 }
 VALU_PEAK_TFLOPS = 157.3
 
