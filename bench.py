@@ -303,6 +303,7 @@ def time_multiscene(eng, steps, warmup):
             v, pv, fo = camera_uniforms(eye, w, h, up=(0.0, 0.0, 1.0))
             views.append((s, v, pv, fo, w, h))
     eng.setViews(views)
+    tuned = eng.autotuneBatch()  # both sorts' block shapes for this layout (forward frames, not timed)
     npix = sum(v[4] * v[5] for v in views)
     dl = np.random.default_rng(5).standard_normal(3 * npix).astype(np.float32)
     d_dl = eng.alloc(dl.nbytes)
@@ -333,7 +334,7 @@ def time_multiscene(eng, steps, warmup):
     eng.clearScenes()
     return {"scenes": 16, "views": len(views), "gaussians": int(sum(sizes)), "pixels": int(npix), "pairs": P,
             "ms_per_step_wall": round(wall * 1000.0, 4), "ms_per_step_gpu": round(med(gpu), 4),
-            "views_per_s": round(len(views) / wall, 2)}
+            "views_per_s": round(len(views) / wall, 2), "autotune": tuned}
 
 
 def hbm_ceiling(eng):
