@@ -418,11 +418,38 @@ __device__ __forceinline__ void blend_bwd_tile(BlendBwdLds& L, const TileView& t
                     v[4] = dL_dpower * (-0.5f * dy * dy);
                 }
             }
-            // row sums by DPP, then one LDS atomic per row of 16 lanes
+            // Row sums.  v[0..7] by a reduce-scatter: lanes exchange half of their values with the
+            // lane across bit 0, then across bit 1 (quad_perm), keeping the halves whose index bits
+            // match their own, and the four quads of the row add up by rotations; lane (b0, b1) of
+            // every quad then holds the row sums of values 4 b0 + 2 b1 + {0, 1}, and the first quad
+            // adds them into the tile's float64 sums (2 LDS atomics per row instead of 8).  v[8] by a
+            // plain row reduction.  The order of every addition is fixed.
+            const bool b0 = (lane & 1u) != 0u, b1 = (lane & 2u) != 0u;
+            float x[4];
 #pragma unroll
-            for (uint32_t q = 0; q < kG; ++q) {
-                const float sum = row_sum16(v[q]);
-                if ((lane & 15u) == 15u && sum != 0.0f && j < kTilePixels) atomicAdd(&L.acc[j][q], (double)sum);
+            for (uint32_t k = 0; k < 4; ++k) {
+                const float send = b0 ? v[k] : v[4 + k], keep = b0 ? v[4 + k] : v[k];
+                x[k] = keep + dpp_f32<0xB1>(send);  // quad_perm [1, 0, 3, 2]
+            }
+            float y[2];
+#pragma unroll
+            for (uint32_t m = 0; m < 2; ++m) {
+                const float send = b1 ? x[m] : x[2 + m], keep = b1 ? x[2 + m] : x[m];
+                y[m] = keep + dpp_f32<0x4E>(send);  // quad_perm [2, 3, 0, 1]
+            }
+#pragma unroll
+            for (uint32_t m = 0; m < 2; ++m) {
+                y[m] += dpp_f32<0x124>(y[m]);  // row_ror:4
+                y[m] += dpp_f32<0x128>(y[m]);  // row_ror:8
+            }
+            const float s8 = row_sum16(v[8]);
+            if (j < kTilePixels) {
+                if ((lane & 12u) == 0u) {
+                    const uint32_t q0 = (b0 ? 4u : 0u) + (b1 ? 2u : 0u);
+                    if (y[0] != 0.0f) atomicAdd(&L.acc[j][q0], (double)y[0]);
+                    if (y[1] != 0.0f) atomicAdd(&L.acc[j][q0 + 1u], (double)y[1]);
+                }
+                if ((lane & 15u) == 15u && s8 != 0.0f) atomicAdd(&L.acc[j][8], (double)s8);
             }
         };
         for (int32_t i = (int32_t)walk - 1; i >= 0; i -= 2) {

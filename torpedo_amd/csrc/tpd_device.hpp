@@ -447,6 +447,13 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
+// The value of another lane of the wave by a DPP move (CTRL: quad_perm, row rotation, ...; every
+// lane must be active).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
 // Sum over each row of 16 lanes with DPP moves only: quad_perm [1,0,3,2] and [2,3,0,1] -> quad
 // sums, row_half_mirror -> 8-lane sums, row_mirror -> every lane holds its row's sum.
 __device__ __forceinline__ float row_sum16(float v) {
