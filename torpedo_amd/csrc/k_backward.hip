@@ -50,10 +50,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(const PreBackwardArgs a)
 #pragma unroll
     for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
     if (tiles > 0) {
-        const float* src = a.partials + (size_t)a.offsets[i] * 9;
-        for (uint32_t t = 0; t < tiles; ++t)
-#pragma unroll
-            for (int q = 0; q < 9; ++q) gr[q] += src[t * 9 + q];
+        sum_pair_partials(a.partials + (size_t)a.offsets[i] * 9, tiles, gr);
         const uint32_t e = a.ent_idx ? a.ent_idx[i] : 0u;
         accumulate_param_grads(a.gauss + (size_t)i * 60, a.ent[e].vm, a.ent[e].pvm, a.cam, a.sh_degree, gr, gm, gls,
                                gq, gop, gsh);

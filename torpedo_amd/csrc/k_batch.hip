@@ -587,10 +587,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
         float gr[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
-        const float* src = a.partials + (size_t)a.offsets[item] * 9;
-        for (uint32_t t = 0; t < tiles; ++t)
-#pragma unroll
-            for (int q = 0; q < 9; ++q) gr[q] += src[t * 9 + q];
+        sum_pair_partials(a.partials + (size_t)a.offsets[item] * 9, tiles, gr);
         const EntityMats& m = mats[vw.mat_base + e];
         float vm[3] = {0.f, 0.f, 0.f};
         accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, vm, gls, gq, gop, gsh);

@@ -10,6 +10,25 @@
 
 namespace tpd {
 
+// gr[q] += the tiles' partial sums src[t * 9 + q], t = 0 .. tiles - 1, in that order (the sums are
+// the same floats as a loop over t).  Four tiles' 36 loads are issued together: one at a time, each
+// pair waited for its loads, and a lane walked as many pairs as the largest footprint of its wave.
+__device__ __forceinline__ void sum_pair_partials(const float* __restrict__ src, uint32_t tiles, float gr[9]) {
+    uint32_t t = 0;
+    for (; t + 4u <= tiles; t += 4u) {
+        float x[36];
+#pragma unroll
+        for (int k = 0; k < 36; ++k) x[k] = src[t * 9u + (uint32_t)k];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) gr[q] += x[u * 9 + q];
+    }
+    for (; t < tiles; ++t)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) gr[q] += src[t * 9u + (uint32_t)q];
+}
+
 // Adds this view's contribution to gm (means), gls (log-scales), gq (quaternion), gop (opacity
 // logit) and gsh (48 SH coefficients).  g is the Gaussian record (60 floats), VM / PVM the entity's
 // view-model and projection-view-model matrices, gr the summed dL/d(image x, y, conic a, b, c,
