@@ -191,6 +191,7 @@ def time_plain(eng, g, eye, w, h, steps, warmup, tune=False):
     dt = time.perf_counter() - t0
     P = eng.pairCount
     out = {k: med([s[k] for s in stages]) for k in stages[0]}
+    out["blend_evaluations"] = eng.blendEvaluations()  # the last frame, blended once more, not timed
     if tuned is not None:
         out["autotune"] = tuned
     return dt, P, out

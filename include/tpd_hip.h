@@ -146,7 +146,8 @@ int tpd_memcpy(tpd_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kin
  * blend and whole-frame times in milliseconds; *passes the number of radix passes. */
 int tpd_set_profiling(tpd_ctx* ctx, int enable);
 int tpd_get_stage_times(tpd_ctx* ctx, float* ms, uint32_t* passes);
-/* Pixel-splat pairs whose power the blend evaluated in the last profiled frame. */
+/* Pixel-splat pairs whose power the blend of the last rendered frame evaluates (counted by blending
+ * that frame once more with a counting kernel; same image). */
 int tpd_get_blend_evaluations(tpd_ctx* ctx, uint64_t* evaluations);
 /* Time of one plain device-to-device copy of `bytes` (bandwidth calibration), averaged over iters. */
 int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per_copy);

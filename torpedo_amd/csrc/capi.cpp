@@ -432,10 +432,21 @@ int tpd_get_stage_times(tpd_ctx* c, float* ms, uint32_t* passes) {
     return TPD_OK;
 }
 
+// The last rendered frame is blended once more by the counting kernel: the same image, transmittance
+// and last contributors are written again, and the frame's own blend runs uncounted.
 int tpd_get_blend_evaluations(tpd_ctx* c, uint64_t* evaluations) {
     if (!check_ctx(c) || !evaluations) return fail(TPD_ERR_INVALID, "tpd_get_blend_evaluations: bad argument");
-    if (!c->profile || !c->rendered || !c->d_eval_count)
-        return fail(TPD_ERR_STATE, "tpd_get_blend_evaluations: no profiled frame");
+    if (!c->rendered || !c->has_blend) return fail(TPD_ERR_STATE, "tpd_get_blend_evaluations: nothing rendered");
+    hipSetDevice(c->device);
+    hipError_t e = hipSuccess;
+    if (!c->d_eval_count && (e = hipMalloc(&c->d_eval_count, sizeof(uint64_t))) != hipSuccess)
+        return hip_fail(e, "alloc evaluation counter");
+    if ((e = hipMemsetAsync(c->d_eval_count, 0, sizeof(uint64_t), c->stream)) != hipSuccess)
+        return hip_fail(e, "clear evaluation counter");
+    BlendArgs b = c->last_blend;
+    b.eval_count = c->d_eval_count;
+    launch_blend(b, c->stream);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "counting blend");
     return copy_back(c, evaluations, c->d_eval_count, sizeof(uint64_t));
 }
 

@@ -113,12 +113,6 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     c->sorted_keys = in_alt ? c->d_keys_alt : c->d_keys;
     c->sorted_vals = in_alt ? c->d_vals_alt : c->d_vals;
     const uint32_t nd = (c->n + kPreBlock - 1) / kPreBlock;
-    if (c->profile) {
-        if (!c->d_eval_count && (e = hipMalloc(&c->d_eval_count, sizeof(uint64_t))) != hipSuccess)
-            return hip_fail(e, "alloc evaluation counter");
-        if ((e = hipMemsetAsync(c->d_eval_count, 0, sizeof(uint64_t), s)) != hipSuccess)
-            return hip_fail(e, "clear evaluation counter");
-    }
     launch_ranges(c->sorted_keys, c->d_counters, P, ntiles, c->d_ranges, c->d_scan_desc, nd,
                   &c->d_counters[kCtrScanTicket], s, 0xFFFFFFFFu, c->key_drop, c->lo_bits);
     launch_tile_order(c->d_ranges, ntiles, nullptr, c->d_tile_order, s);
@@ -134,10 +128,12 @@ int render_forward(tpd_ctx* c, hipStream_t s) {
     b.ntiles = ntiles;
     b.image = c->d_image;
     b.rgba8 = c->d_rgba8;
-    b.eval_count = c->profile ? c->d_eval_count : nullptr;
+    b.eval_count = nullptr;  // counted on request, by blending the frame once more
     b.final_T = c->d_final_T;  // kept for a backward of this frame
     b.final_last = c->d_final_last;
     launch_blend(b, s);
+    c->last_blend = b;
+    c->has_blend = true;
     if (c->profile) hipEventRecord(c->ev[6], s);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "forward launch");

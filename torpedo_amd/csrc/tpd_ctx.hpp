@@ -72,7 +72,9 @@ struct tpd_ctx {
     // stage timing with HIP events on the render stream (tpd_set_profiling)
     bool profile = false;
     hipEvent_t ev[8] = {};
-    uint64_t* d_eval_count = nullptr;  // blend evaluations of the last profiled frame
+    uint64_t* d_eval_count = nullptr;  // blend evaluations, counted on request (tpd_get_blend_evaluations)
+    tpd::BlendArgs last_blend{};       // the last rendered frame's blend, for that count
+    bool has_blend = false;
 
     // backward and batched state (backward.cpp, batch.cpp)
     tpd::BackwardState* bwd = nullptr;

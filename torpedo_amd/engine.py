@@ -456,10 +456,14 @@ class GaussianEngine:
         check(lib().tpd_get_stage_times(self._ctx, ptr(ms), ctypes.byref(passes)))
         out = {k: float(v) for k, v in zip(self.STAGES, ms)}
         out["radix_passes"] = int(passes.value)
+        return out
+
+    def blendEvaluations(self) -> int:
+        """Pixel-splat pairs whose power the last rendered frame's blend evaluates
+        (tpd_get_blend_evaluations: the frame is blended once more by a counting kernel)."""
         ev = ctypes.c_uint64()
         check(lib().tpd_get_blend_evaluations(self._ctx, ctypes.byref(ev)))
-        out["blend_evaluations"] = int(ev.value)
-        return out
+        return int(ev.value)
 
     def measureCopy(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
         """GB/s of a plain device copy (read + write bytes counted)."""
