@@ -207,6 +207,8 @@ def time_fwd_bwd(eng, steps, warmup):
     eng.uploadRecords(g, 3)
     view, pv, fo = camera_uniforms([0.0, 0.0, -8.0], W, H)
     eng.setCamera(view, pv, fo, W, H)
+    eng.render()
+    tuned = eng.autotune()  # the radix shape for this frame's keys (not timed)
     eng.setProfiling(True)
     up = np.random.default_rng(3).standard_normal((H, W, 3)).astype(np.float32)
     d_up = eng.alloc(up.nbytes)
@@ -230,7 +232,8 @@ def time_fwd_bwd(eng, steps, warmup):
     eng.setProfiling(False)
     m = {k: round(med([r[k] for r in rec]), 4) for k in rec[0]}
     return {"ms": m["fwd_bwd_span"], "wall_ms_per_step": round(wall, 4), "pairs": P, "gaussians": n,
-            "width": W, "height": H, "stage_ms": {k: v for k, v in m.items() if k not in ("radix_passes",)}}
+            "width": W, "height": H, "stage_ms": {k: v for k, v in m.items() if k not in ("radix_passes",)},
+            "autotune": tuned}
 
 
 def pipeline_cameras(n, w=1920, h=1080):
