@@ -549,28 +549,33 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
                                                                     const uint32_t* __restrict__ block_scene,
                                                                     const EntityMats* __restrict__ mats,
                                                                     const BatchGradsDev* __restrict__ grads) {
-    // Each thread's record is read from HBM once and kept in LDS for the views of its scene: read
-    // again per view, the records of the resident blocks (about 6 MB per XCD) outgrow the XCD's L2
-    // and every view fetched them anew.  A stride of 61 floats keeps the lanes' reads of one field
-    // on distinct banks; each thread reads only its own record, so no barrier is needed.
+    // The block's records (consecutive Gaussians, 240 B each) are read from HBM once, as consecutive
+    // 16-byte loads of the whole block (one thread's record at a time would make every load touch
+    // 64 cache lines), and kept in LDS for the views of their scene: read again per view, the
+    // records of the resident blocks (about 6 MB per XCD) outgrow the XCD's L2.  A stride of 61
+    // floats keeps the lanes' reads of one field on distinct banks.  The same LDS later holds the
+    // block's SH gradients (48 floats per Gaussian), stored to HBM the same way.
     constexpr uint32_t kRecStride = 61;
-    __shared__ float s_rec[kPreBlock * kRecStride];
+    __shared__ alignas(16) float s_rec[kPreBlock * kRecStride];
     if (a.counters[kCtrOverflow]) return;  // gradients are left untouched; the step is run again
     const uint32_t b = blockIdx.x;
     const uint32_t si = block_scene[b];
     const BatchSceneDev& sc = scenes[si];
-    const uint32_t g = (b - sc.first_block) * kPreBlock + threadIdx.x;
-    if (g >= sc.n) return;
-    float* rec = s_rec + threadIdx.x * kRecStride;
+    const uint32_t g0 = (b - sc.first_block) * kPreBlock;
+    const uint32_t g = g0 + threadIdx.x;
+    const uint32_t nvalid = min(kPreBlock, sc.n - g0);  // >= 1: the block exists for these Gaussians
+    const bool valid = threadIdx.x < nvalid;           // every thread stays for the barriers
     {
-        const float4* src = reinterpret_cast<const float4*>(sc.gauss + (size_t)g * 60);
-#pragma unroll
-        for (int q = 0; q < 15; ++q) {
-            const float4 v = src[q];
-            rec[4 * q + 0] = v.x; rec[4 * q + 1] = v.y; rec[4 * q + 2] = v.z; rec[4 * q + 3] = v.w;
+        const float4* src = reinterpret_cast<const float4*>(sc.gauss + (size_t)g0 * 60);
+        for (uint32_t x = threadIdx.x; x < nvalid * 15u; x += kPreBlock) {
+            const float4 v = src[x];
+            float* d = s_rec + (x / 15u) * kRecStride + 4u * (x % 15u);
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
         }
     }
-    const uint32_t e = sc.ent_idx ? sc.ent_idx[g] : 0u;
+    __syncthreads();
+    const float* rec = s_rec + threadIdx.x * kRecStride;
+    const uint32_t e = (valid && sc.ent_idx) ? sc.ent_idx[g] : 0u;
     // The same FP32 operations as a plain backward per view accumulated into the gradient buffers:
     // every log-scale, quaternion, opacity and SH entry receives one term per view, added to the
     // running sum directly; a mean receives three terms per view, so they are summed on their own
@@ -579,24 +584,35 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
     float gsh[48];
 #pragma unroll
     for (int k = 0; k < 48; ++k) gsh[k] = 0.f;
-    for (uint32_t k = 0; k < sc.nviews; ++k) {
-        const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
-        const uint32_t item = vw.item_base + g;
-        const uint32_t tiles = a.tiles[item];
-        if (tiles == 0) continue;
-        float gr[9];
+    if (valid) {
+        for (uint32_t k = 0; k < sc.nviews; ++k) {
+            const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
+            const uint32_t item = vw.item_base + g;
+            const uint32_t tiles = a.tiles[item];
+            if (tiles == 0) continue;
+            float gr[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
-        sum_pair_partials(a.partials + (size_t)a.offsets[item] * 9, tiles, gr);
-        const EntityMats& m = mats[vw.mat_base + e];
-        float vm[3] = {0.f, 0.f, 0.f};
-        accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, vm, gls, gq, gop, gsh);
+            for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
+            sum_pair_partials(a.partials + (size_t)a.offsets[item] * 9, tiles, gr);
+            const EntityMats& m = mats[vw.mat_base + e];
+            float vm[3] = {0.f, 0.f, 0.f};
+            accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, vm, gls, gq, gop, gsh);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) gm[c] += vm[c];
+            for (int c = 0; c < 3; ++c) gm[c] += vm[c];
+        }
     }
     const BatchGradsDev& gd = grads[si];
-    store_param_grads(g, a.accumulate != 0, gm, gls, gq, gop, gsh, gd.means, gd.log_scales, gd.quats,
-                      gd.opacity_logits, gd.sh);
+    if (valid) store_param_grads_but_sh(g, a.accumulate != 0, gm, gls, gq, gop, gd.means, gd.log_scales, gd.quats,
+                                        gd.opacity_logits);
+    __syncthreads();  // the records are done with: the LDS takes the SH gradients
+    if (valid) {
+        float4* d = reinterpret_cast<float4*>(s_rec + threadIdx.x * 48u);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) d[k] = make_float4(gsh[4 * k], gsh[4 * k + 1], gsh[4 * k + 2], gsh[4 * k + 3]);
+    }
+    __syncthreads();
+    store_sh_grads_block(reinterpret_cast<const float4*>(s_rec), nvalid, a.accumulate != 0,
+                         reinterpret_cast<float4*>(gd.sh + (size_t)g0 * 48));
 }
 
 void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s) {

@@ -228,6 +228,37 @@ __device__ __forceinline__ void accumulate_param_grads(const float* g, const flo
 }
 
 // Writes (or adds, when accumulate) one Gaussian's parameter gradients.
+// store_param_grads without the SH gradients (the batch stores those for a whole block at once).
+__device__ __forceinline__ void store_param_grads_but_sh(uint32_t i, bool accumulate, const float gm[3],
+                                                         const float gls[3], const float gq[4], float gop,
+                                                         float* grad_means, float* grad_log_scales, float* grad_quats,
+                                                         float* grad_opacity_logits) {
+    float* om = grad_means + (size_t)i * 3;
+    float* os = grad_log_scales + (size_t)i * 3;
+    float* oq = grad_quats + (size_t)i * 4;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        om[c] = accumulate ? om[c] + gm[c] : gm[c];
+        os[c] = accumulate ? os[c] + gls[c] : gls[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) oq[c] = accumulate ? oq[c] + gq[c] : gq[c];
+    grad_opacity_logits[i] = accumulate ? grad_opacity_logits[i] + gop : gop;
+}
+
+// The SH gradients of n consecutive Gaussians (48 floats each, src in LDS, Gaussian-major) to dst,
+// as consecutive 16-byte stores by the whole block (accumulate: dst + src, as store_param_grads).
+__device__ __forceinline__ void store_sh_grads_block(const float4* src, uint32_t n, bool accumulate, float4* dst) {
+    for (uint32_t x = threadIdx.x; x < n * 12u; x += blockDim.x) {
+        float4 v = src[x];
+        if (accumulate) {
+            const float4 o = dst[x];
+            v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+        }
+        dst[x] = v;
+    }
+}
+
 __device__ __forceinline__ void store_param_grads(uint32_t i, bool accumulate, const float gm[3], const float gls[3],
                                                   const float gq[4], float gop, const float gsh[48], float* grad_means,
                                                   float* grad_log_scales, float* grad_quats, float* grad_opacity_logits,
