@@ -21,6 +21,8 @@
 #include "tpd_grad.hpp"
 #include "tpd_kernels.hpp"
 
+#include <type_traits>
+
 namespace tpd {
 
 // Also clears the frame's depth-window flag, which the count pass that follows may set.
@@ -584,7 +586,9 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
     float gsh[48];
 #pragma unroll
     for (int k = 0; k < 48; ++k) gsh[k] = 0.f;
-    if (valid) {
+    // ent: the scene has entities; without, every Gaussian uses the view's first entity matrices,
+    // a block-uniform address (scalar loads)
+    const auto views_of = [&](auto ent) {
         for (uint32_t k = 0; k < sc.nviews; ++k) {
             const BatchViewDev& vw = views[scene_views[sc.view_begin + k]];
             const uint32_t item = vw.item_base + g;
@@ -594,12 +598,18 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_preprocess_bwd(const BatchP
 #pragma unroll
             for (int q = 0; q < 9; ++q) gr[q] = 0.0f;
             sum_pair_partials(a.partials + (size_t)a.offsets[item] * 9, tiles, gr);
-            const EntityMats& m = mats[vw.mat_base + e];
+            const EntityMats& m = mats[vw.mat_base + (decltype(ent)::value ? e : 0u)];
             float vm[3] = {0.f, 0.f, 0.f};
             accumulate_param_grads(rec, m.vm, m.pvm, vw.cam, sc.sh_degree, gr, vm, gls, gq, gop, gsh);
 #pragma unroll
             for (int c = 0; c < 3; ++c) gm[c] += vm[c];
         }
+    };
+    if (valid) {
+        if (sc.ent_idx)
+            views_of(std::true_type{});
+        else
+            views_of(std::false_type{});
     }
     const BatchGradsDev& gd = grads[si];
     if (valid) store_param_grads_but_sh(g, a.accumulate != 0, gm, gls, gq, gop, gd.means, gd.log_scales, gd.quats,
