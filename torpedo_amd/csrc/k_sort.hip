@@ -21,6 +21,7 @@ namespace tpd {
 constexpr uint32_t kFlagA = 1u << 30;
 constexpr uint32_t kFlagP = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;  // kWideDrop: the rank of a dropped key
 
 template <uint32_t BITS, typename KT>
 __device__ __forceinline__ uint32_t digit_of(KT k, uint32_t shift) {
@@ -167,8 +168,8 @@ __global__ __launch_bounds__(256) void k_sort_offsets(uint32_t* __restrict__ cou
 // the 32-bit key (compacted key >> drop), possible once the passes before it have consumed the bits
 // below `drop` and the rest fit in 32 bits; kNarrowed reads and writes such 32-bit keys.
 // kWideDrop is kWide that also drops the keys whose high word is zero (the batch's culled items,
-// which the digit counts leave out): they are ranked after every digit and never written.  Its
-// values are the keys' indices (the batch's items), generated rather than read.
+// which the digit counts leave out): they take no part in the ranking, get no LDS slot and are
+// never written.  Its values are the keys' indices (the batch's items), generated rather than read.
 enum : uint32_t { kWide = 0, kNarrow = 1, kNarrowed = 2, kWideDrop = 3 };
 
 struct OnesweepArgs {
@@ -199,8 +200,7 @@ struct OnesweepLds {
     uint32_t bstart[R];
     uint32_t adj[R];
     uint32_t wtot[8];
-    uint32_t wcull[8];          // kWideDrop: per wave dropped keys (then wave offsets among them)
-    uint32_t nkeep;             // kWideDrop: the partition's kept keys (the dropped ones follow)
+    uint32_t nkeep;             // kWideDrop: the partition's kept keys
     uint32_t part;
     KT keys[kTileKeys];         // the partition reordered by digit (then its values)
 };
@@ -263,7 +263,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) L.part = atomicAdd(a.ticket, 1u);
     for (uint32_t x = tid; x < kWaves * R / 2; x += BLOCK) reinterpret_cast<uint32_t*>(&L.whist[0][0])[x] = 0;
-    if (DROP && tid < 8) L.wcull[tid] = 0;
     __syncthreads();
     const uint32_t part = __builtin_amdgcn_readfirstlane(L.part);  // block-uniform: keep it scalar
     if (part >= nparts) return;
@@ -323,17 +322,13 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
             peers &= bit ? bal : ~bal;
         }
         if constexpr (DROP) {
-            // the dropped keys of the wave form one more group, after every digit
+            // a dropped key is nobody's peer and gets no rank (kNoSlot)
             const bool cull = dropped(k[j]);
-            const uint64_t cb = ballot(cull);
-            peers = cull ? cb : (peers & ~cb);
+            peers &= ~ballot(cull);
             const uint32_t below = (uint32_t)__popcll(peers & lt);
-            const uint32_t pre = cull ? L.wcull[wave] : (uint32_t)L.whist[wave][d];
-            if (below == 0) {
-                if (cull) L.wcull[wave] = pre + (uint32_t)__popcll(peers);
-                else L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
-            }
-            rank[j] = pre + below;
+            const uint32_t pre = L.whist[wave][d];
+            if (!cull && below == 0) L.whist[wave][d] = (uint16_t)(pre + (uint32_t)__popcll(peers));
+            rank[j] = cull ? kNoSlot : pre + below;
         } else {
             const uint32_t below = (uint32_t)__popcll(peers & lt);
             const uint32_t pre = L.whist[wave][d];
@@ -342,14 +337,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
         }
     }
     __syncthreads();
-    if (DROP && tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t w = 0; w < kWaves; ++w) {
-            const uint32_t c = L.wcull[w];
-            L.wcull[w] = acc;
-            acc += c;
-        }
-    }
 
     // Per digit: wave exclusive offsets and the partition's digit counts (published); then the
     // digits' block-local starts by an exclusive scan over the digit-owning threads.
@@ -391,7 +378,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 #pragma unroll
     for (uint32_t j = 0; j < ITEMS; ++j) {
         const uint32_t d = digit(k[j]);
-        const uint32_t lp = dropped(k[j]) ? L.nkeep + L.wcull[wave] + rank[j] : L.bstart[d] + L.whist[wave][d] + rank[j];
+        if (DROP && rank[j] == kNoSlot) continue;
+        const uint32_t lp = L.bstart[d] + L.whist[wave][d] + rank[j];
         rank[j] = lp;
         L.keys[lp] = k[j];
     }
@@ -413,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
 
     // Keys out, each to its digit's global start + its place in the partition; the global position
     // of each output slot is kept for the values (the keys' LDS is reused for them).
-    // (the kept keys come first in LDS: the past-the-end keys are in the last digit, or dropped)
+    // (the past-the-end keys are in the last digit, after the others; dropped keys have no slot)
     const uint32_t nvalid = DROP ? L.nkeep : min(kTileKeys, n - base);
     const bool all_valid = full && !DROP;
     uint32_t gpos[ITEMS];
@@ -427,7 +415,8 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const OnesweepArgs a) {
     __syncthreads();
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(L.keys);
 #pragma unroll
-    for (uint32_t j = 0; j < ITEMS; ++j) s_vals[rank[j]] = v[j];
+    for (uint32_t j = 0; j < ITEMS; ++j)
+        if (!DROP || rank[j] != kNoSlot) s_vals[rank[j]] = v[j];
     __syncthreads();
 #pragma unroll
     for (uint32_t t = 0; t < ITEMS; ++t) {
