@@ -74,7 +74,7 @@ def multiview_views(scene_id=0):
 
 # Radix-pass block shapes in the library's table order: (digit bits, threads, keys per thread).
 SORT_SHAPES = [(8, 256, 16), (8, 256, 8), (8, 256, 12), (8, 512, 8), (8, 512, 12), (8, 256, 20), (9, 512, 12),
-               (9, 512, 8), (9, 512, 16)]
+               (9, 512, 8), (9, 512, 16), (9, 1024, 16), (9, 1024, 12), (8, 1024, 12)]
 
 
 def shape_desc(i):
@@ -362,6 +362,34 @@ def hbm_ceiling(eng):
     return {"hbm_ceiling_gbs": out, "random_reads_g_per_s": gathers, "peak_gbs": HBM_PEAK_GBS}
 
 
+def run_scatter(eng):
+    """The memory pattern of a radix pass without its ranking (tpd_measure_run_scatter), at the
+    sizes and block shapes of the sorts it bounds: keys and values read coalesced and written into
+    per-(digit, block) runs, with 1-4 resident blocks per CU or no cap.  Each case gives GB/s
+    (bytes read + written) and the time per pass."""
+    cases = [  # (name, key bytes, keys, digit bins, keys per block)
+        ("batch pair pass (58.3M, 32-bit keys, 9-bit digits, 512 x 16)", 4, 58_288_090, 512, 8192),
+        ("batch item pass (24.2M kept, 64-bit keys, 9-bit digits, 512 x 12)", 8, 24_166_852, 512, 6144),
+        ("plain 1080p 32-bit pass (21.1M, 8-bit digits, 512 x 12)", 4, 21_122_859, 256, 6144),
+        ("plain 1080p 64-bit pass (21.1M, 8-bit digits, 512 x 12)", 8, 21_122_859, 256, 6144),
+    ]
+    for kpb in (4096, 6144, 12288, 16384):  # run length: 8 .. 32 keys per (digit, block)
+        cases.append((f"pair-sized sweep (58.3M, 32-bit keys, 9-bit digits, {kpb} keys per block)", 4, 58_288_090,
+                      512, kpb))
+    out = []
+    for name, kb, n, bins, kpb in cases:
+        n = n // kpb * kpb
+        for bpc in (0, 1, 2, 3, 4):
+            gbs = eng.measureRunScatter(kb, n, bins, kpb, bpc)
+            us = 2.0 * n * (kb + 4) / (gbs * 1e9) * 1e6
+            out.append({"case": name, "keys": n, "key_bytes": kb, "bins": bins, "keys_per_block": kpb,
+                        "keys_per_run": kpb / bins, "blocks_per_cu": bpc, "gbs": round(gbs, 1),
+                        "us_per_pass": round(us, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 3)})
+            log(f"[bench] run scatter {name}, {bpc or 'any'} blocks/CU: {out[-1]['gbs']} GB/s, "
+                f"{out[-1]['us_per_pass']} us")
+    return {"run_scatter": out, "peak_gbs": HBM_PEAK_GBS}
+
+
 def sort_vs_rocprim():
     """tests/cpp/sort_vs_rocprim (built by __graft_entry__.build()): the stand-alone sort under every
     block shape against rocPRIM's radix_sort_pairs, same keys, values and key bits."""
@@ -414,7 +442,7 @@ def main():
     ap.add_argument("--sort-shape", type=int, default=-1,
                     help="fixed radix-pass block shape instead of autotuning (for profiler runs)")
     ap.add_argument("--only", choices=["multiview", "plain", "sort-load", "fwd-bwd", "multiscene", "pipelined",
-                                       "hbm-ceiling", "sort-vs-rocprim"],
+                                       "hbm-ceiling", "run-scatter", "sort-vs-rocprim"],
                     default=None,
                     help="run one leg only (for profiler runs)")
     args = ap.parse_args()
@@ -459,6 +487,10 @@ def main():
         return
     if args.only == "hbm-ceiling":
         print(json.dumps(hbm_ceiling(eng)), flush=True)
+        eng.close()
+        return
+    if args.only == "run-scatter":
+        print(json.dumps(run_scatter(eng)), flush=True)
         eng.close()
         return
     if args.only == "sort-vs-rocprim":

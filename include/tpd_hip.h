@@ -156,6 +156,15 @@ int tpd_measure_copy(tpd_ctx* ctx, uint64_t bytes, uint32_t iters, float* ms_per
  * (0: 8192 blocks of 256 threads).  Kinds 5 and 6: 2^25 random 8-byte / 4-byte reads from an array
  * of `bytes` (the rate of a gather by a sorted index).  *ms is the mean time per pass over iters. */
 int tpd_measure_bandwidth(tpd_ctx* ctx, uint32_t kind, uint64_t bytes, uint32_t iters, uint32_t blocks, float* ms);
+/* The memory pattern of one stable radix pass without its ranking (the ceiling of a pass's data
+ * movement): n keys of key_bytes (4 or 8) and n 4-byte values, in blocks of keys_per_block (a
+ * multiple of 512, at most 16,384) whose keys each carry one of `bins` (256 or 512) random digits,
+ * are read coalesced and written into per-(digit, block) runs at a stable pass's positions,
+ * consecutive lanes storing consecutive places of one run.  n is a multiple of keys_per_block.
+ * blocks_per_cu > 0 caps the blocks resident per CU (by unused LDS), 0 leaves them to the
+ * registers.  *ms is the mean time per pass over iters. */
+int tpd_measure_run_scatter(tpd_ctx* ctx, uint32_t key_bytes, uint32_t n, uint32_t bins, uint32_t keys_per_block,
+                            uint32_t blocks_per_cu, uint32_t iters, float* ms);
 
 /* ----- stand-alone stable radix sort (radix-*.slang, GaussianEngine.cpp:819-841) -----
  * Sorts n (key, value) pairs by the key's low key_bits bits (stable); bits at and above key_bits do

@@ -84,7 +84,7 @@ def test_sort_empty_and_invalid(eng):
         gpu_sort(eng, np.zeros(4, np.uint64), np.zeros(4, np.uint32), 0)
 
 
-NUM_SHAPES = 9  # 8-bit digits x 6 block shapes, 9-bit digits x 3
+NUM_SHAPES = 12  # 8-bit digits x 7 block shapes, 9-bit digits x 5
 
 
 def test_shape_count(eng):

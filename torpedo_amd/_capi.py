@@ -73,6 +73,7 @@ SIGNATURES = {
     "tpd_get_blend_evaluations": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "tpd_measure_copy": (_i32, [_vp, ctypes.c_uint64, _u32, ctypes.POINTER(_f32)]),
     "tpd_measure_bandwidth": (_i32, [_vp, _u32, ctypes.c_uint64, _u32, _u32, ctypes.POINTER(_f32)]),
+    "tpd_measure_run_scatter": (_i32, [_vp, _u32, _u32, _u32, _u32, _u32, _u32, ctypes.POINTER(_f32)]),
     "tpd_sort_pairs": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(_i32)]),
     "tpd_sort_pairs_noalloc": (_i32, [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, ctypes.POINTER(_i32)]),
     "tpd_autotune": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),

@@ -441,7 +441,7 @@ struct SortShape {
 static const SortShape kShapes[] = {
     TPD_SHAPE(8, 256, 16), TPD_SHAPE(8, 256, 8),  TPD_SHAPE(8, 256, 12), TPD_SHAPE(8, 512, 8),
     TPD_SHAPE(8, 512, 12), TPD_SHAPE(8, 256, 20), TPD_SHAPE(9, 512, 12), TPD_SHAPE(9, 512, 8),
-    TPD_SHAPE(9, 512, 16),
+    TPD_SHAPE(9, 512, 16), TPD_SHAPE(9, 1024, 16), TPD_SHAPE(9, 1024, 12), TPD_SHAPE(8, 1024, 12),
 };
 #undef TPD_SHAPE
 constexpr uint32_t kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);

@@ -80,9 +80,152 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ src, uint3
     out[i0 + 3] = d;
 }
 
+// The memory pattern of one stable radix pass without its ranking: block b reads its keys_per_block
+// keys and values coalesced, already in digit order inside the block (key & (bins - 1) is the
+// digit), and writes each at run_start[b][d] + (j - local_start[b][d]), so that consecutive lanes
+// store consecutive places of one digit run, as a pass writes its LDS-reordered keys.  Each thread
+// loads all its keys_per_block / 512 (at most kRunScatterItems) keys and values before it stores
+// any, as a pass holds its keys in registers.  The tables (bins words each per block) come from
+// random digit histograms; they are read into LDS once per block.
+constexpr uint32_t kRunScatterItems = 32;
+template <typename K>
+__global__ __launch_bounds__(512) void k_run_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                     const uint32_t* __restrict__ run_start,
+                                                     const uint32_t* __restrict__ local_start, uint32_t keys_per_block,
+                                                     uint32_t bins) {
+    __shared__ uint32_t s_run[512], s_local[512];
+    const size_t b = blockIdx.x;
+    for (uint32_t d = threadIdx.x; d < bins; d += 512) {
+        s_run[d] = run_start[b * bins + d];
+        s_local[d] = local_start[b * bins + d];
+    }
+    const size_t base = b * keys_per_block;
+    const uint32_t items = keys_per_block / 512;
+    K k[kRunScatterItems];
+    uint32_t v[kRunScatterItems];
+#pragma unroll
+    for (uint32_t i = 0; i < kRunScatterItems; ++i)
+        if (i < items) {
+            k[i] = kin[base + threadIdx.x + i * 512];
+            v[i] = vin[base + threadIdx.x + i * 512];
+        }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < kRunScatterItems; ++i)
+        if (i < items) {
+            const uint32_t j = threadIdx.x + i * 512;
+            const uint32_t d = (uint32_t)k[i] & (bins - 1);
+            const size_t dst = (size_t)s_run[d] + (j - s_local[d]);
+            kout[dst] = k[i];
+            vout[dst] = v[i];
+        }
+}
+
 }  // namespace tpd
 
 using namespace tpd;
+
+extern "C" int tpd_measure_run_scatter(tpd_ctx* c, uint32_t key_bytes, uint32_t n, uint32_t bins,
+                                       uint32_t keys_per_block, uint32_t blocks_per_cu, uint32_t iters, float* ms) {
+    if (!c || !ms || iters == 0 || (key_bytes != 4 && key_bytes != 8) || (bins != 256 && bins != 512) ||
+        keys_per_block == 0 || keys_per_block % 512 || keys_per_block > 512 * kRunScatterItems || n == 0 ||
+        n % keys_per_block)
+        return fail(TPD_ERR_INVALID, "tpd_measure_run_scatter: bad argument");
+    hipSetDevice(c->device);
+    const uint32_t nblocks = n / keys_per_block;
+    // Residency limit: dynamic LDS so that blocks_per_cu blocks fit in a CU's LDS and one more
+    // does not (0: no limit beyond the registers).
+    int lds_cu = 0, lds_block = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+        return fail(TPD_ERR_HIP, "tpd_measure_run_scatter: LDS size query");
+    const size_t lds_static = 2 * 512 * sizeof(uint32_t);
+    size_t lds_dyn = 0;
+    if (blocks_per_cu) {
+        lds_dyn = ((size_t)lds_cu / (blocks_per_cu + 1) + 1024 - lds_static + 255) / 256 * 256;
+        if (lds_dyn + lds_static > (size_t)lds_block || (lds_dyn + lds_static) * blocks_per_cu > (size_t)lds_cu)
+            return fail(TPD_ERR_INVALID, "tpd_measure_run_scatter: blocks_per_cu outside what the LDS can express");
+    }
+    try {
+        // Random digits per key (a 64-bit mix of its index), counted per block; the block's keys are
+        // laid out in digit order, so only the histogram matters.
+        const size_t tn = (size_t)nblocks * bins;
+        std::vector<uint32_t> counts(tn, 0), run(tn), local(tn);
+        std::vector<uint32_t> keys32(key_bytes == 4 ? n : 0);
+        std::vector<uint64_t> keys64(key_bytes == 8 ? n : 0);
+        for (uint32_t blk = 0; blk < nblocks; ++blk) {
+            uint32_t* cnt = &counts[(size_t)blk * bins];
+            for (uint32_t j = 0; j < keys_per_block; ++j) {
+                uint64_t z = ((uint64_t)blk * keys_per_block + j + 1) * 0x9E3779B97F4A7C15ull;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                ++cnt[(uint32_t)(z ^ (z >> 31)) & (bins - 1)];
+            }
+            uint32_t at = 0;
+            for (uint32_t d = 0; d < bins; ++d) {
+                local[(size_t)blk * bins + d] = at;
+                for (uint32_t r = 0; r < cnt[d]; ++r, ++at) {
+                    const size_t i = (size_t)blk * keys_per_block + at;
+                    if (key_bytes == 4) keys32[i] = d | ((uint32_t)i << 9);
+                    else keys64[i] = d | ((uint64_t)i << 9);
+                }
+            }
+        }
+        // Global positions: digit-major, then block order (the stable order of a radix pass).
+        uint64_t at = 0;
+        for (uint32_t d = 0; d < bins; ++d)
+            for (uint32_t blk = 0; blk < nblocks; ++blk) {
+                run[(size_t)blk * bins + d] = (uint32_t)at;
+                at += counts[(size_t)blk * bins + d];
+            }
+        if (at != n) return fail(TPD_ERR_INVALID, "tpd_measure_run_scatter: histograms do not cover the keys");
+        void *kin = nullptr, *kout = nullptr;
+        uint32_t *vin = nullptr, *vout = nullptr, *d_run = nullptr, *d_local = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const size_t kb = (size_t)n * key_bytes, tb = (size_t)nblocks * bins * 4;
+        hipError_t e = hipMalloc(&kin, kb);
+        if (e == hipSuccess) e = hipMalloc(&kout, kb);
+        if (e == hipSuccess) e = hipMalloc(&vin, (size_t)n * 4);
+        if (e == hipSuccess) e = hipMalloc(&vout, (size_t)n * 4);
+        if (e == hipSuccess) e = hipMalloc(&d_run, tb);
+        if (e == hipSuccess) e = hipMalloc(&d_local, tb);
+        if (e == hipSuccess)
+            e = hipMemcpy(kin, key_bytes == 4 ? (const void*)keys32.data() : (const void*)keys64.data(), kb,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_run, run.data(), tb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_local, local.data(), tb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemsetAsync(vin, 0, (size_t)n * 4, c->stream);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        auto one = [&]() -> hipError_t {
+            if (key_bytes == 4)
+                hipLaunchKernelGGL(k_run_scatter<uint32_t>, dim3(nblocks), dim3(512), lds_dyn, c->stream,
+                                   static_cast<const uint32_t*>(kin), vin, static_cast<uint32_t*>(kout), vout, d_run,
+                                   d_local, keys_per_block, bins);
+            else
+                hipLaunchKernelGGL(k_run_scatter<uint64_t>, dim3(nblocks), dim3(512), lds_dyn, c->stream,
+                                   static_cast<const uint64_t*>(kin), vin, static_cast<uint64_t*>(kout), vout, d_run,
+                                   d_local, keys_per_block, bins);
+            return hipGetLastError();
+        };
+        if (e == hipSuccess) e = one();  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        for (uint32_t i = 0; i < iters && e == hipSuccess; ++i) e = one();
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        *ms = t / (float)iters;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void* p : {kin, kout, (void*)vin, (void*)vout, (void*)d_run, (void*)d_local})
+            if (p) (void)hipFree(p);
+        return e == hipSuccess ? TPD_OK : hip_fail(e, "tpd_measure_run_scatter");
+    } catch (const std::bad_alloc&) {
+        return fail(TPD_ERR_CAPACITY, "tpd_measure_run_scatter: host memory for the key layout");
+    }
+}
 
 // kind: 0 copy kernel, 1 hipMemcpyAsync device to device, 2 nontemporal copy kernel, 3 read-only
 // kernel, 4 write-only kernel, 5 / 6 random 8-byte / 4-byte reads (2^25 of them from an array of

@@ -491,6 +491,17 @@ class GaussianEngine:
                                           ctypes.byref(ms)))
         return (1 << 25) / (ms.value * 1e-3) / 1e9
 
+    def measureRunScatter(self, key_bytes: int, n: int, bins: int, keys_per_block: int, blocks_per_cu: int = 0,
+                          iters: int = 20) -> float:
+        """GB/s (bytes read + written) of a radix pass's memory pattern without its ranking
+        (tpd_measure_run_scatter): n keys of key_bytes and n 4-byte values, blocks of keys_per_block
+        keys with random digits among `bins`, written into per-(digit, block) runs; blocks_per_cu
+        caps the resident blocks per CU (0: no cap)."""
+        ms = ctypes.c_float()
+        check(lib().tpd_measure_run_scatter(self._ctx, int(key_bytes), int(n), int(bins), int(keys_per_block),
+                                            int(blocks_per_cu), int(iters), ctypes.byref(ms)))
+        return 2.0 * n * (key_bytes + 4) / (ms.value * 1e-3) / 1e9
+
     def deviceImagePtr(self) -> int:
         p = ctypes.c_void_p()
         check(lib().tpd_device_image(self._ctx, ctypes.byref(p)))
