@@ -41,6 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_onesweep launch of the batched frame (mean over its launches), measured with
 # FETCH_SIZE x 2 + WRITE_SIZE for the (item sort, pair sort) block shapes named; other shapes: null.
 ONESWEEP_TRAFFIC = {
+    (10, 9): {"bytes_per_launch": 779_400_000,
+              "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 614.7 MB, later "
+                        "2 x 624.9 MB; pair passes 2 x 1,016.3 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
     (6, 8): {"bytes_per_launch": 893_200_000,
              "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 731.9 MB, later "
                        "2 x 708.0 MB; pair passes 2 x 1,159.1 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
@@ -365,21 +368,24 @@ def hbm_ceiling(eng):
 def run_scatter(eng):
     """The memory pattern of a radix pass without its ranking (tpd_measure_run_scatter), at the
     sizes and block shapes of the sorts it bounds: keys and values read coalesced and written into
-    per-(digit, block) runs, with 1-4 resident blocks per CU or no cap.  Each case gives GB/s
-    (bytes read + written) and the time per pass."""
-    cases = [  # (name, key bytes, keys, digit bins, keys per block)
-        ("batch pair pass (58.3M, 32-bit keys, 9-bit digits, 512 x 16)", 4, 58_288_090, 512, 8192),
-        ("batch item pass (24.2M kept, 64-bit keys, 9-bit digits, 512 x 12)", 8, 24_166_852, 512, 6144),
-        ("plain 1080p 32-bit pass (21.1M, 8-bit digits, 512 x 12)", 4, 21_122_859, 256, 6144),
-        ("plain 1080p 64-bit pass (21.1M, 8-bit digits, 512 x 12)", 8, 21_122_859, 256, 6144),
+    per-(digit, block) runs with uniformly random digits.  Each case gives GB/s (bytes read +
+    written) and the time per pass; the pair case also with 1, 2 and 4 resident blocks per CU."""
+    cases = [  # (name, key bytes, keys, digit bins, keys per block, blocks per CU to try)
+        ("batch pair pass (58.3M, 32-bit keys, 9-bit digits, 1024 x 16)", 4, 58_288_090, 512, 16384, (0, 1, 2, 4)),
+        ("batch pair pass (58.3M, 32-bit keys, 9-bit digits, 512 x 16)", 4, 58_288_090, 512, 8192, (0,)),
+        ("batch item pass (24.2M kept, 64-bit keys, 9-bit digits, 1024 x 12)", 8, 24_166_852, 512, 12288, (0,)),
+        ("batch item pass (24.2M kept, 64-bit keys, 9-bit digits, 512 x 12)", 8, 24_166_852, 512, 6144, (0,)),
+        ("plain 1080p 32-bit pass (21.1M, 8-bit digits, 1024 x 12)", 4, 21_122_859, 256, 12288, (0,)),
+        ("plain 1080p 64-bit pass (21.1M, 8-bit digits, 1024 x 12)", 8, 21_122_859, 256, 12288, (0,)),
+        ("plain 1080p 32-bit pass (21.1M, 8-bit digits, 512 x 12)", 4, 21_122_859, 256, 6144, (0,)),
     ]
-    for kpb in (4096, 6144, 12288, 16384):  # run length: 8 .. 32 keys per (digit, block)
+    for kpb in (4096, 6144, 12288):  # run length 8, 12, 24 keys per (digit, block); 16 and 32 above
         cases.append((f"pair-sized sweep (58.3M, 32-bit keys, 9-bit digits, {kpb} keys per block)", 4, 58_288_090,
-                      512, kpb))
+                      512, kpb, (0,)))
     out = []
-    for name, kb, n, bins, kpb in cases:
+    for name, kb, n, bins, kpb, caps in cases:
         n = n // kpb * kpb
-        for bpc in (0, 1, 2, 3, 4):
+        for bpc in caps:
             gbs = eng.measureRunScatter(kb, n, bins, kpb, bpc)
             us = 2.0 * n * (kb + 4) / (gbs * 1e9) * 1e6
             out.append({"case": name, "keys": n, "key_bytes": kb, "bins": bins, "keys_per_block": kpb,

@@ -160,7 +160,8 @@ int tpd_measure_bandwidth(tpd_ctx* ctx, uint32_t kind, uint64_t bytes, uint32_t 
  * movement): n keys of key_bytes (4 or 8) and n 4-byte values, in blocks of keys_per_block (a
  * multiple of 512, at most 16,384) whose keys each carry one of `bins` (256 or 512) random digits,
  * are read coalesced and written into per-(digit, block) runs at a stable pass's positions,
- * consecutive lanes storing consecutive places of one run.  n is a multiple of keys_per_block.
+ * consecutive lanes storing consecutive places of one run, all keys before the values (blocks of
+ * 512 threads, 1,024 from 12,288 keys per block).  n is a multiple of keys_per_block.
  * blocks_per_cu > 0 caps the blocks resident per CU (by unused LDS), 0 leaves them to the
  * registers.  *ms is the mean time per pass over iters. */
 int tpd_measure_run_scatter(tpd_ctx* ctx, uint32_t key_bytes, uint32_t n, uint32_t bins, uint32_t keys_per_block,

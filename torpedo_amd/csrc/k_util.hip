@@ -83,43 +83,49 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ src, uint3
 // The memory pattern of one stable radix pass without its ranking: block b reads its keys_per_block
 // keys and values coalesced, already in digit order inside the block (key & (bins - 1) is the
 // digit), and writes each at run_start[b][d] + (j - local_start[b][d]), so that consecutive lanes
-// store consecutive places of one digit run, as a pass writes its LDS-reordered keys.  Each thread
-// loads all its keys_per_block / 512 (at most kRunScatterItems) keys and values before it stores
-// any, as a pass holds its keys in registers.  The tables (bins words each per block) come from
-// random digit histograms; they are read into LDS once per block.
+// store consecutive places of one digit run, as a pass writes its LDS-reordered keys: first all
+// keys, then all values.  Each thread loads its keys_per_block / blockDim.x (at most
+// kRunScatterItems) keys and values before it stores any, as a pass holds them in registers.  The
+// tables (bins words each per block) come from random digit histograms; they are read into LDS
+// once per block.
 constexpr uint32_t kRunScatterItems = 32;
 template <typename K>
-__global__ __launch_bounds__(512) void k_run_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                     K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                     const uint32_t* __restrict__ run_start,
-                                                     const uint32_t* __restrict__ local_start, uint32_t keys_per_block,
-                                                     uint32_t bins) {
+__global__ __launch_bounds__(1024) void k_run_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                      K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                      const uint32_t* __restrict__ run_start,
+                                                      const uint32_t* __restrict__ local_start,
+                                                      uint32_t keys_per_block, uint32_t bins) {
     __shared__ uint32_t s_run[512], s_local[512];
     const size_t b = blockIdx.x;
-    for (uint32_t d = threadIdx.x; d < bins; d += 512) {
+    const uint32_t nt = blockDim.x;
+    for (uint32_t d = threadIdx.x; d < bins; d += nt) {
         s_run[d] = run_start[b * bins + d];
         s_local[d] = local_start[b * bins + d];
     }
     const size_t base = b * keys_per_block;
-    const uint32_t items = keys_per_block / 512;
+    const uint32_t items = keys_per_block / nt;
     K k[kRunScatterItems];
     uint32_t v[kRunScatterItems];
 #pragma unroll
     for (uint32_t i = 0; i < kRunScatterItems; ++i)
+        if (i < items) k[i] = kin[base + threadIdx.x + i * nt];
+#pragma unroll
+    for (uint32_t i = 0; i < kRunScatterItems; ++i)
+        if (i < items) v[i] = vin[base + threadIdx.x + i * nt];
+    __syncthreads();
+    uint32_t dst[kRunScatterItems];
+#pragma unroll
+    for (uint32_t i = 0; i < kRunScatterItems; ++i)
         if (i < items) {
-            k[i] = kin[base + threadIdx.x + i * 512];
-            v[i] = vin[base + threadIdx.x + i * 512];
+            const uint32_t j = threadIdx.x + i * nt;
+            const uint32_t d = (uint32_t)k[i] & (bins - 1);
+            dst[i] = s_run[d] + (j - s_local[d]);
+            kout[dst[i]] = k[i];
         }
     __syncthreads();
 #pragma unroll
     for (uint32_t i = 0; i < kRunScatterItems; ++i)
-        if (i < items) {
-            const uint32_t j = threadIdx.x + i * 512;
-            const uint32_t d = (uint32_t)k[i] & (bins - 1);
-            const size_t dst = (size_t)s_run[d] + (j - s_local[d]);
-            kout[dst] = k[i];
-            vout[dst] = v[i];
-        }
+        if (i < items) vout[dst[i]] = v[i];
 }
 
 }  // namespace tpd
@@ -134,6 +140,8 @@ extern "C" int tpd_measure_run_scatter(tpd_ctx* c, uint32_t key_bytes, uint32_t 
         return fail(TPD_ERR_INVALID, "tpd_measure_run_scatter: bad argument");
     hipSetDevice(c->device);
     const uint32_t nblocks = n / keys_per_block;
+    // 512 threads per block, 1,024 from 12,288 keys per block (the radix passes' shapes)
+    const uint32_t threads = keys_per_block >= 12288 && keys_per_block % 1024 == 0 ? 1024 : 512;
     // Residency limit: dynamic LDS so that blocks_per_cu blocks fit in a CU's LDS and one more
     // does not (0: no limit beyond the registers).
     int lds_cu = 0, lds_block = 0;
@@ -200,11 +208,11 @@ extern "C" int tpd_measure_run_scatter(tpd_ctx* c, uint32_t key_bytes, uint32_t 
         if (e == hipSuccess) e = hipEventCreate(&e1);
         auto one = [&]() -> hipError_t {
             if (key_bytes == 4)
-                hipLaunchKernelGGL(k_run_scatter<uint32_t>, dim3(nblocks), dim3(512), lds_dyn, c->stream,
+                hipLaunchKernelGGL(k_run_scatter<uint32_t>, dim3(nblocks), dim3(threads), lds_dyn, c->stream,
                                    static_cast<const uint32_t*>(kin), vin, static_cast<uint32_t*>(kout), vout, d_run,
                                    d_local, keys_per_block, bins);
             else
-                hipLaunchKernelGGL(k_run_scatter<uint64_t>, dim3(nblocks), dim3(512), lds_dyn, c->stream,
+                hipLaunchKernelGGL(k_run_scatter<uint64_t>, dim3(nblocks), dim3(threads), lds_dyn, c->stream,
                                    static_cast<const uint64_t*>(kin), vin, static_cast<uint64_t*>(kout), vout, d_run,
                                    d_local, keys_per_block, bins);
             return hipGetLastError();
