@@ -119,8 +119,8 @@ def test_batch_views_of_one_scene(engines, orc):
 
 def test_batch_emission_of_large_footprints(engines, orc):
     """Pair emission stages the pairs of items with at most four tiles in LDS, chunk by chunk of a
-    block's output range (2,048 sorted items per block, chunks of 3,840 pairs), and writes them
-    directly when the block's larger items push that range past four chunks (15,360 pairs).  Small
+    block's output range (4,096 sorted items per block, chunks of 7,680 pairs), and writes them
+    directly when the block's larger items push that range past four chunks (30,720 pairs).  Small
     and large splats mixed in depth order exercise both paths and chunk boundaries that cut through
     items; every view must equal its plain frame."""
     beng, plain = engines
@@ -139,8 +139,8 @@ def test_batch_emission_of_large_footprints(engines, orc):
     beng.syncBatch()
     P = beng.batchPairCount
     say(f"[batch] large footprints: {P} pairs over {g.shape[0] * len(views)} items")
-    blocks = -(-g.shape[0] * len(views) // 2048)  # emission blocks of 2,048 depth-sorted items
-    assert P > 15360 * blocks  # so that some block's range is longer than four chunks
+    blocks = -(-g.shape[0] * len(views) // 4096)  # emission blocks of 4,096 depth-sorted items
+    assert P > 30720 * blocks  # so that some block's range is longer than four chunks
     check_batch(beng, plain, [g], [3], views, "large footprints", orc=orc)
 
 

@@ -252,15 +252,16 @@ void launch_batch_items(const BatchArgs& a, hipStream_t s) {
 // the block (pair j of them belongs to the item whose inclusive count of large tiles first exceeds
 // j).  The last partition stores the pair total and the overflow flag.
 constexpr uint32_t kEmitItems = kEmitBlockItems;
-constexpr uint32_t kEmitPerThread = kEmitItems / kPreBlock;
+constexpr uint32_t kEmitThreads = 512;
+constexpr uint32_t kEmitPerThread = kEmitItems / kEmitThreads;
 static_assert(kEmitPerThread % 4 == 0, "items are loaded four at a time");
 
 // Pairs of the block's small rectangles staged per chunk before they are written out: the large
 // items' tables (start, rectangle; their values are re-read from the sorted items) are done with
 // by then and lend their space.
-// 2 x 2,048 - 256 pairs: with the large items' tables (24 KB) in the same space, the kernel's LDS
-// stays under 40 KB, four blocks per CU.
-constexpr uint32_t kEmitStage = 2 * kEmitItems - kPreBlock;
+// 2 x 4,096 - 512 pairs: with the large items' tables (48 KB) in the same space, the kernel's LDS
+// stays under 80 KB, two blocks of 512 threads per CU.
+constexpr uint32_t kEmitStage = 2 * kEmitItems - kEmitThreads;
 constexpr uint32_t kEmitMaxChunks = 4;
 constexpr uint32_t kNoPair = 0xFFFFFFFFu;  // an empty staging slot (no item has this index)
 union EmitLds {
@@ -273,9 +274,9 @@ union EmitLds {
     } stage;
 };
 
-__global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
+__global__ __launch_bounds__(kEmitThreads) void k_batch_emit(const BatchArgs a) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
+    __shared__ uint32_t s_wave_sum[kEmitThreads / kWave], s_wave_big[kEmitThreads / kWave];
     __shared__ uint32_t s_big_incl[kEmitItems];
     __shared__ EmitLds L;
     __shared__ uint64_t s_base;
@@ -285,7 +286,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     // the pair sort's counts come from the item kernel: its first pass's descriptors (which the
     // item sort's last pass may have used) and the tickets are cleared here
-    for (uint32_t x = blockIdx.x * kPreBlock + tid; x < a.pair_desc0_words; x += gridDim.x * kPreBlock)
+    for (uint32_t x = blockIdx.x * kEmitThreads + tid; x < a.pair_desc0_words; x += gridDim.x * kEmitThreads)
         a.hist_desc0[x] = 0u;
     if (a.pair_desc0_words && blockIdx.x == 0 && tid < kMaxPasses) a.hist_tickets[tid] = 0u;
     // The sorted items are the visible ones (the item sort dropped the culled).  The grid is sized
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     __syncthreads();
     uint32_t wave_off = 0, block_total = 0, big_off = 0, big_total = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+    for (uint32_t w = 0; w < kEmitThreads / kWave; ++w) {
         const uint32_t v = s_wave_sum[w], vb = s_wave_big[w];
         wave_off += (w < wave) ? v : 0u;
         block_total += v;
@@ -399,13 +400,13 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
     for (uint32_t c0 = 0; c0 < block_total; c0 += kEmitStage) {
         const uint32_t n = min(block_total - c0, kEmitStage);
         __syncthreads();  // the large items' tables, or the previous chunk, are done with
-        for (uint32_t p = tid; p < n; p += kPreBlock) L.stage.val[p] = kNoPair;
+        for (uint32_t p = tid; p < n; p += kEmitThreads) L.stage.val[p] = kNoPair;
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kEmitPerThread; ++j)
             if (t[j] && t[j] <= kOwnTiles) stage_own_rect(t[j], rect[j], first[j], c0, n, L.stage.key, L.stage.val, item[j]);
         __syncthreads();
-        for (uint32_t p = tid; p < n; p += kPreBlock) {
+        for (uint32_t p = tid; p < n; p += kEmitThreads) {
             const uint32_t v = L.stage.val[p];
             const uint64_t g = base + c0 + p;
             if (v != kNoPair && g < a.cap) {
@@ -420,9 +421,9 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_emit(const BatchArgs a) {
 // where the backward keeps each item's per-pair partial sums (consecutive Gaussians of a view then
 // have adjacent partials, which the parameter-gradient pass reads item by item).  kEmitPerThread
 // items per thread, decoupled look-back over the second half of the scan descriptors.
-__global__ __launch_bounds__(kPreBlock) void k_batch_item_offsets(const BatchArgs a) {
+__global__ __launch_bounds__(kEmitThreads) void k_batch_item_offsets(const BatchArgs a) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave];
+    __shared__ uint32_t s_wave_sum[kEmitThreads / kWave];
     __shared__ uint64_t s_base;
     const uint32_t tid = threadIdx.x, wave = tid / kWave, lane = __lane_id();
     if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket2], 1u);
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_item_offsets(const BatchArg
     __syncthreads();
     uint32_t wave_off = 0, block_total = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+    for (uint32_t w = 0; w < kEmitThreads / kWave; ++w) {
         const uint32_t v = s_wave_sum[w];
         wave_off += (w < wave) ? v : 0u;
         block_total += v;
@@ -486,13 +487,13 @@ __global__ __launch_bounds__(kPreBlock) void k_batch_item_offsets(const BatchArg
 void launch_batch_item_offsets(const BatchArgs& a, hipStream_t s) {
     const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
     if (nparts == 0 || !a.offsets || !a.tiles) return;
-    hipLaunchKernelGGL(k_batch_item_offsets, dim3(nparts), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_item_offsets, dim3(nparts), dim3(kEmitThreads), 0, s, a);
 }
 
 void launch_batch_emit(const BatchArgs& a, hipStream_t s) {
     const uint32_t nparts = (a.nitems + kEmitItems - 1) / kEmitItems;
     if (nparts == 0) return;
-    hipLaunchKernelGGL(k_batch_emit, dim3(nparts), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_batch_emit, dim3(nparts), dim3(kEmitThreads), 0, s, a);
 }
 
 // ------------------------------------------------------------------ blend, forward and backward

@@ -688,7 +688,7 @@ __device__ __forceinline__ void emit_large_rects(const uint32_t* s_big_incl, con
                                                  uint32_t n,
                                                  uint32_t big_total, uint64_t base, uint64_t cap,
                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    for (uint32_t j = threadIdx.x; j < big_total; j += kPreBlock) {
+    for (uint32_t j = threadIdx.x; j < big_total; j += blockDim.x) {
         uint32_t lo = 0, hi = n - 1;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;

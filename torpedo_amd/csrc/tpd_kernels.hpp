@@ -255,7 +255,7 @@ void launch_batch_mats(const BatchSceneDev* scenes, const BatchViewDev* views, u
 void launch_batch_items(const BatchArgs& a, hipStream_t s);
 void launch_batch_emit(const BatchArgs& a, hipStream_t s);
 void launch_batch_item_offsets(const BatchArgs& a, hipStream_t s);
-constexpr uint32_t kEmitBlockItems = 2048;  // sorted items per emission partition (k_batch_emit)
+constexpr uint32_t kEmitBlockItems = 4096;  // sorted items per emission partition (k_batch_emit)
 void launch_batch_blend(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_blend_bwd(const BatchBlendArgs& a, hipStream_t s);
 void launch_batch_preprocess_bwd(const BatchPreBwdArgs& a, hipStream_t s);
