@@ -41,9 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_onesweep launch of the batched frame (mean over its launches), measured with
 # FETCH_SIZE x 2 + WRITE_SIZE for the (item sort, pair sort) block shapes named; other shapes: null.
 ONESWEEP_TRAFFIC = {
-    (10, 9): {"bytes_per_launch": 779_400_000,
-              "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 614.7 MB, later "
-                        "2 x 624.9 MB; pair passes 2 x 1,016.3 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
+    (10, 9): {"bytes_per_launch": 779_200_000,
+              "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 614.2 MB, later "
+                        "2 x 626.1 MB; pair passes 2 x 1,014.9 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
     (6, 8): {"bytes_per_launch": 893_200_000,
              "source": "profiles/batch_frame-multiview32-clusters1M.md (item passes: first 731.9 MB, later "
                        "2 x 708.0 MB; pair passes 2 x 1,159.1 MB; FETCH_SIZE x 2 + WRITE_SIZE)"},
