@@ -25,24 +25,27 @@ void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams&
 
 // ------------------------------------------------------------------ preprocess + scan + duplicate
 
-__global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs a) {
+// Gaussians per preprocess partition: one per thread, 512 threads (half the look-backs of 256).
+constexpr uint32_t kDupBlock = 512;
+
+__global__ __launch_bounds__(kDupBlock) void k_preprocess_dup(const ForwardArgs a) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_sum[kPreBlock / kWave], s_wave_big[kPreBlock / kWave];
-    __shared__ uint32_t s_big_incl[kPreBlock], s_start[kPreBlock];
-    __shared__ int32_t s_x0[kPreBlock], s_y0[kPreBlock], s_w[kPreBlock];
-    __shared__ uint32_t s_depth[kPreBlock];
+    __shared__ uint32_t s_wave_sum[kDupBlock / kWave], s_wave_big[kDupBlock / kWave];
+    __shared__ uint32_t s_big_incl[kDupBlock], s_start[kDupBlock];
+    __shared__ int32_t s_x0[kDupBlock], s_y0[kDupBlock], s_w[kDupBlock];
+    __shared__ uint32_t s_depth[kDupBlock];
     __shared__ uint64_t s_base;
 
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_part = atomicAdd(&a.counters[kCtrScanTicket], 1u);
     // Tile ranges are cleared here (every block strides over them) so that the ranges kernel,
     // which runs after the sort, only writes the non-empty tiles.
-    for (uint32_t t = blockIdx.x * kPreBlock + tid; t < a.ntiles; t += gridDim.x * kPreBlock)
+    for (uint32_t t = blockIdx.x * kDupBlock + tid; t < a.ntiles; t += gridDim.x * kDupBlock)
         a.ranges[t] = make_uint2(0u, 0u);
     __syncthreads();
     const uint32_t part = __builtin_amdgcn_readfirstlane(s_part);  // block-uniform: keep it scalar
     if (part >= gridDim.x) return;  // tickets are reset every frame; never true in a consistent state
-    const uint32_t i = part * kPreBlock + tid;
+    const uint32_t i = part * kDupBlock + tid;
 
     Projected p;
     p.tiles = 0;
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
     __syncthreads();
     uint32_t wave_off = 0, block_total = 0, big_off = 0, big_total = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kPreBlock / kWave; ++w) {
+    for (uint32_t w = 0; w < kDupBlock / kWave; ++w) {
         const uint32_t v = s_wave_sum[w], vb = s_wave_big[w];
         wave_off += (w < wave) ? v : 0u;
         block_total += v;
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
         if (lane == 0) {
             if (part > 0) store_relaxed(&a.scan_desc[part], kDescP | (excl + block_total));
             s_base = excl;
-            const uint32_t nparts = (a.n + kPreBlock - 1) / kPreBlock;
+            const uint32_t nparts = (a.n + kDupBlock - 1) / kDupBlock;
             if (part == nparts - 1) {
                 const uint64_t total = excl + block_total;
                 a.counters[kCtrPairs] = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
@@ -112,14 +115,14 @@ __global__ __launch_bounds__(kPreBlock) void k_preprocess_dup(const ForwardArgs 
     if (t && !big)
         emit_own_pairs(t, p.x0, p.y0, p.x1, as_uint(p.vz), base + excl_in_block, a.gx, 0u, a.cap, a.keys, a.vals,
                        a.val_base + i);
-    emit_large_pairs(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, a.gx, 0u, a.cap, a.keys, a.vals,
-                     a.val_base + part * kPreBlock);
+    emit_large_pairs<kDupBlock>(s_big_incl, s_start, s_x0, s_y0, s_w, s_depth, big_total, base, a.gx, 0u, a.cap, a.keys, a.vals,
+                     a.val_base + part * kDupBlock);
 }
 
 void launch_preprocess_dup(const ForwardArgs& a, hipStream_t s) {
-    const uint32_t nb = (a.n + kPreBlock - 1) / kPreBlock;
+    const uint32_t nb = (a.n + kDupBlock - 1) / kDupBlock;
     if (nb == 0) return;
-    hipLaunchKernelGGL(k_preprocess_dup, dim3(nb), dim3(kPreBlock), 0, s, a);
+    hipLaunchKernelGGL(k_preprocess_dup, dim3(nb), dim3(kDupBlock), 0, s, a);
 }
 
 // ------------------------------------------------------------------ ranges

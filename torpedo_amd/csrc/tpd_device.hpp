@@ -578,14 +578,15 @@ __device__ __forceinline__ void emit_own_pairs(uint32_t tiles, int32_t x0, int32
     }
 }
 
+template <uint32_t NB>  // threads of the block, one Gaussian each
 __device__ __forceinline__ void emit_large_pairs(const uint32_t* s_big_incl, const uint32_t* s_start,
                                                  const int32_t* s_x0, const int32_t* s_y0, const int32_t* s_w,
                                                  const uint32_t* s_depth, uint32_t big_total, uint64_t base,
                                                  uint32_t gx, uint32_t tile_base, uint64_t cap,
                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                  uint32_t val_first) {
-    for (uint32_t j = threadIdx.x; j < big_total; j += kPreBlock) {
-        uint32_t lo = 0, hi = kPreBlock - 1;
+    for (uint32_t j = threadIdx.x; j < big_total; j += NB) {
+        uint32_t lo = 0, hi = NB - 1;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (s_big_incl[mid] > j) hi = mid; else lo = mid + 1;
