@@ -25,8 +25,9 @@ void launch_entity_mats(const float* models, uint32_t n_ent, const CameraParams&
 
 // ------------------------------------------------------------------ preprocess + scan + duplicate
 
-// Gaussians per preprocess partition: one per thread, 512 threads (half the look-backs of 256).
-constexpr uint32_t kDupBlock = 512;
+// Gaussians per preprocess partition: one per thread, 1,024 threads (a quarter of the look-backs of
+// 256; 63 VGPRs and 24 KB of LDS keep two blocks per CU).
+constexpr uint32_t kDupBlock = 1024;
 
 __global__ __launch_bounds__(kDupBlock) void k_preprocess_dup(const ForwardArgs a) {
     __shared__ uint32_t s_part;
