@@ -227,8 +227,8 @@ __device__ __forceinline__ void accumulate_param_grads(const float* g, const flo
     gm[2] += (gdz - z * dot) * il;
 }
 
-// Writes (or adds, when accumulate) one Gaussian's parameter gradients.
-// store_param_grads without the SH gradients (the batch stores those for a whole block at once).
+// Writes (or adds, when accumulate) one Gaussian's parameter gradients but the SH ones, which the
+// parameter-gradient passes store for a whole block at once (store_sh_grads_block).
 __device__ __forceinline__ void store_param_grads_but_sh(uint32_t i, bool accumulate, const float gm[3],
                                                          const float gls[3], const float gq[4], float gop,
                                                          float* grad_means, float* grad_log_scales, float* grad_quats,
@@ -247,7 +247,7 @@ __device__ __forceinline__ void store_param_grads_but_sh(uint32_t i, bool accumu
 }
 
 // The SH gradients of n consecutive Gaussians (48 floats each, src in LDS, Gaussian-major) to dst,
-// as consecutive 16-byte stores by the whole block (accumulate: dst + src, as store_param_grads).
+// as consecutive 16-byte stores by the whole block (accumulate: dst + src).
 __device__ __forceinline__ void store_sh_grads_block(const float4* src, uint32_t n, bool accumulate, float4* dst) {
     for (uint32_t x = threadIdx.x; x < n * 12u; x += blockDim.x) {
         float4 v = src[x];
@@ -256,39 +256,6 @@ __device__ __forceinline__ void store_sh_grads_block(const float4* src, uint32_t
             v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
         }
         dst[x] = v;
-    }
-}
-
-__device__ __forceinline__ void store_param_grads(uint32_t i, bool accumulate, const float gm[3], const float gls[3],
-                                                  const float gq[4], float gop, const float gsh[48], float* grad_means,
-                                                  float* grad_log_scales, float* grad_quats, float* grad_opacity_logits,
-                                                  float* grad_sh) {
-    float* om = grad_means + (size_t)i * 3;
-    float* os = grad_log_scales + (size_t)i * 3;
-    float* oq = grad_quats + (size_t)i * 4;
-    float* osh = grad_sh + (size_t)i * 48;
-    if (accumulate) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            om[c] += gm[c];
-            os[c] += gls[c];
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) oq[c] += gq[c];
-        grad_opacity_logits[i] += gop;
-#pragma unroll
-        for (int k = 0; k < 48; ++k) osh[k] += gsh[k];
-    } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            om[c] = gm[c];
-            os[c] = gls[c];
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) oq[c] = gq[c];
-        grad_opacity_logits[i] = gop;
-#pragma unroll
-        for (int k = 0; k < 48; ++k) osh[k] = gsh[k];
     }
 }
 
