@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kDupBlock) void k_preprocess_dup(const ForwardArgs 
         const uint32_t e = a.ent_idx ? a.ent_idx[i] : 0u;
         // The record into registers at once, 16-byte loads when the records are 16-byte aligned (read
         // field by field as the projection needs them, each load waited for in turn, the kernel took
-        // a quarter longer at 78 instead of 63 VGPRs).
+        // a quarter longer, at 63 instead of 78 VGPRs).
         const float* g = a.gauss + (size_t)i * 60;
         float rec[60];
         if ((reinterpret_cast<uintptr_t>(a.gauss) & 15u) == 0u) {  // uniform
