@@ -43,7 +43,8 @@ struct ForwardArgs {
     Splat* splats;               // per Gaussian (zeros when culled)
     uint32_t* tiles;             // tile count per Gaussian
     uint32_t* offsets;           // exclusive scan of tiles
-    uint64_t* scan_desc;         // ceil(N/256) look-back descriptors, zero on entry
+    uint64_t* scan_desc;         // look-back descriptors, one per partition of the preprocess (the
+                                 // buffer holds ceil(N/256)), zero on entry
     uint32_t* counters;          // kCtr* block
     uint64_t* keys;
     uint32_t* vals;
